@@ -1,0 +1,78 @@
+"""ctypes binding of the C-ABI in include/fastkpm_hip.h (libfastkpm_hip.so).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libfastkpm_hip.so"
+
+F32, F64, C32, C64 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32 = 0, 1
+
+ERRORS = {-1: "FKPM_E_ARG", -2: "FKPM_E_STATE", -3: "FKPM_E_HIP", -4: "FKPM_E_NODEV",
+          -5: "FKPM_E_UNSUP"}
+
+# Every entry point of include/fastkpm_hip.h with its ctypes signature.
+_I, _D, _P = C.c_int, C.c_double, C.c_void_p
+_IP, _DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
+SIGNATURES = {
+    "fkpm_last_error": (C.c_char_p, []),
+    "fkpm_device_count": (_I, []),
+    "fkpm_dev_open": (_I, [_I, _I, _I, C.POINTER(_P)]),
+    "fkpm_dev_close": (_I, [_P]),
+    "fkpm_stream": (_P, [_P]),
+    "fkpm_energy_scale": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _I, _P, _DP, _DP]),
+    "fkpm_set_H": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _D]),
+    "fkpm_set_R": (_I, [_P, _I, _I, _P]),
+    "fkpm_moments": (_I, [_P, _I, _DP]),
+    "fkpm_set_D_pattern": (_I, [_P, _I, _I, _IP, _IP, _IP]),
+    "fkpm_stoch_matrix": (_I, [_P, _I, _DP, _P]),
+    "fkpm_autodiff_matrix": (_I, [_P, _I, _DP, _P]),
+    "fkpm_get_D": (_I, [_P, _P]),
+    "fkpm_set_timing": (_I, [_P, _I]),
+    "fkpm_get_timing": (_I, [_P, _DP, _IP, _DP, _IP, _DP, _IP]),
+    "fkpm_get_config": (_I, [_P, _IP, _IP, _IP, _IP, _IP]),
+    "fkpm_set_geometry": (_I, [_P, _I, _I, _I]),
+}
+
+_lib = None
+
+
+class FkpmError(RuntimeError):
+    """A C-ABI call returned a nonzero status."""
+
+    def __init__(self, code: int, fn: str, msg: str):
+        super().__init__(f"{fn} -> {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """The loaded device library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; "
+                "g.build()' or python fastkpm_amd/build.py). There is no CPU fallback.")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, fn: str) -> None:
+    if rc != 0:
+        msg = lib().fkpm_last_error().decode(errors="replace")
+        raise FkpmError(rc, fn, msg)
+
+
+def call(fn: str, *args) -> None:
+    check(getattr(lib(), fn)(*args), fn)

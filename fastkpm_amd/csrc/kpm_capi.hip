@@ -1,0 +1,971 @@
+// kpm_capi.hip -- implementation of the C-ABI in include/fastkpm_hip.h.
+//
+// Owns device memory and the HIP stream of one engine handle, validates every
+// shape on the host before a launch, picks the kernel geometry, and sequences
+// the per-step launches of the Chebyshev recursions.  The algorithms follow
+// Engine_CPU (src/engine_cpu.cpp) and Engine_cuSPARSE (src/engine_cusparse.cpp)
+// of the reference; the kernels are in kpm_kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fastkpm_hip.h"
+#include "kpm_dispatch.h"
+// this unit defines the non-template helper kernels
+#define FKD_DEFINE_HELPERS
+#include "kpm_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) return fail(FKPM_E_HIP, "%s failed: %s (%s:%d)", #x,       \
+                                      hipGetErrorString(e_), __FILE__, __LINE__);    \
+  } while (0)
+
+#define CHK(x)              \
+  do {                      \
+    int rc_ = (x);          \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+template <class T>
+struct TypeId {
+  typedef T type;
+};
+
+// Launch a kernel through its host stub with exactly its parameter types.
+template <typename... P>
+int launch(void (*fp)(P...), dim3 grid, dim3 block, hipStream_t st,
+           typename TypeId<P>::type... args) {
+  void* argv[] = {(void*)&args...};
+  hipError_t e = hipLaunchKernel((const void*)fp, grid, block, argv, 0, st);
+  if (e != hipSuccess) return fail(FKPM_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes && p != nullptr) return 0;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (e != hipSuccess) return fail(FKPM_E_HIP, "hipFree: %s", hipGetErrorString(e));
+    }
+    if (n == 0) return 0;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(FKPM_E_HIP, "hipMalloc(%zu bytes): %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+size_t host_elem_bytes(int dtype) {
+  switch (dtype) {
+    case FKPM_F32: return 4;
+    case FKPM_F64: return 8;
+    case FKPM_C32: return 8;
+    case FKPM_C64: return 16;
+  }
+  return 0;
+}
+bool dtype_complex(int dtype) { return dtype == FKPM_C32 || dtype == FKPM_C64; }
+bool dtype_single(int dtype) { return dtype == FKPM_F32 || dtype == FKPM_C32; }
+
+int grid_for(size_t n, int per_block = 256) {
+  size_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (int)g;
+}
+
+// Validate a BSR/COO block pattern on the host.
+int check_pattern(const char* what, int nb, int nnzb, const int* row_ptr, const int* row_idx,
+                  const int* col_idx) {
+  if (nb <= 0 || nnzb < 0) return fail(FKPM_E_ARG, "%s: bad sizes nb=%d nnzb=%d", what, nb, nnzb);
+  if (!row_ptr || (nnzb > 0 && !col_idx))
+    return fail(FKPM_E_ARG, "%s: null pattern array", what);
+  if (row_ptr[0] != 0 || row_ptr[nb] != nnzb)
+    return fail(FKPM_E_ARG, "%s: row_ptr[0]=%d row_ptr[nb]=%d, expected 0 and %d", what,
+                row_ptr[0], row_ptr[nb], nnzb);
+  for (int i = 0; i < nb; i++) {
+    if (row_ptr[i + 1] < row_ptr[i]) return fail(FKPM_E_ARG, "%s: row_ptr not monotone at %d", what, i);
+    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) {
+      if (col_idx[p] < 0 || col_idx[p] >= nb)
+        return fail(FKPM_E_ARG, "%s: col_idx[%d]=%d out of range [0,%d)", what, p, col_idx[p], nb);
+      if (row_idx && row_idx[p] != i)
+        return fail(FKPM_E_ARG, "%s: row_idx[%d]=%d inconsistent with row_ptr (row %d)", what, p,
+                    row_idx[p], i);
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+struct fkpm_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int dtype = FKPM_C64;
+  int prec = FKPM_PREC_FP64;
+  bool cx = true;
+  bool single = false;  // storage is FP32
+
+  // --- H (scaled) ---
+  int nb = 0, b = 0, nnzb = 0;
+  bool h_set = false;
+  DevBuf rp, ci, hv, rowblk, hraw;
+  double avg = 0.0, mag = 1.0;
+  std::vector<int> h_rp, h_ci;
+
+  // --- R and recursion state ---
+  int n = 0, s = 0;
+  bool r_set = false, r_dirty = true;
+  DevBuf rraw;  // R as given (host layout and type)
+  int L = 0, KQ = 0, qp = 0, sp = 0, nwg = 0, chunk = 0, geo_b = 0;
+  int force_L = 0, force_KQ = 0, max_wg = 0;
+  DevBuf R, A[3], Bv[3], xi, partial, psum;
+  int alpha_M = 0;  // M of the moments whose final state is held, 0 if none
+
+  // --- D ---
+  int nbD = 0, nnzbD = 0;
+  bool d_set = false, d_sym = false, d_fused = false, d_valid = false;
+  DevBuf drp, dci, tpos, dacc, dout;
+  std::vector<int> hd_rp, hd_ci;
+
+  // --- timing ---
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double ms_mom = 0, ms_stoch = 0, ms_ad = 0;
+  int n_mom = 0, n_stoch = 0, n_ad = 0;
+
+  fkd::KSet<double> kd;
+  fkd::KSet<float> kf;
+
+  ~fkpm_handle() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    DevBuf* bufs[] = {&rp, &ci, &hv, &rowblk, &hraw, &rraw, &R, &A[0], &A[1], &A[2], &Bv[0],
+                      &Bv[1], &Bv[2], &xi, &partial, &psum, &drp, &dci, &tpos, &dacc, &dout};
+    for (DevBuf* d : bufs) d->release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  size_t vbytes() const { return (size_t)n * sp * (cx ? 2 : 1) * (single ? 4 : 8); }
+  size_t accbytes() const { return (size_t)n * sp * (cx ? 2 : 1) * 8; }
+  int ne() const { return (single ? 4 : 2) / (cx ? 2 : 1); }  // elements per 16-byte packet
+  void* vec_a(int k) { return k == 0 ? R.p : A[k % 3].p; }  // forward-phase alpha_k
+};
+
+namespace {
+
+int find_kset(fkpm_handle* h, int B, int L, int KQ) {
+  bool ok;
+  if (h->single)
+    ok = h->cx ? fkd::find_kset_fc(B, L, KQ, &h->kf) : fkd::find_kset_fr(B, L, KQ, &h->kf);
+  else
+    ok = h->cx ? fkd::find_kset_dc(B, L, KQ, &h->kd) : fkd::find_kset_dr(B, L, KQ, &h->kd);
+  if (!ok) return fail(FKPM_E_UNSUP, "no kernel for block %d, lanes %d, packets %d", B, L, KQ);
+  return 0;
+}
+
+// Choose (L, KQ): the smallest compiled L*KQ that covers the row of packets in
+// one pass (the fused D reduction then runs once per block), preferring more
+// packets per lane; a forced geometry wins.
+int choose_geometry(fkpm_handle* h) {
+  const int ne = h->ne();
+  const int qp0 = (h->s + ne - 1) / ne;
+  fkd::GeomChoice gs[32];
+  const int ng = fkd::geometries_for_block(h->b, gs, 32);
+  if (ng == 0) return fail(FKPM_E_UNSUP, "block size %d is not supported", h->b);
+  int L = 0, KQ = 0;
+  if (h->force_L > 0) {
+    L = h->force_L;
+    KQ = h->force_KQ > 0 ? h->force_KQ : 1;
+  } else {
+    int best = -1;
+    for (int k = 0; k < ng; k++) {
+      const int w = gs[k].L * gs[k].KQ;
+      if (w >= qp0) {
+        if (best < 0 || w < gs[best].L * gs[best].KQ ||
+            (w == gs[best].L * gs[best].KQ && gs[k].KQ > gs[best].KQ))
+          best = k;
+      }
+    }
+    if (best < 0) {  // several passes: widest geometry
+      best = 0;
+      for (int k = 1; k < ng; k++)
+        if (gs[k].L * gs[k].KQ > gs[best].L * gs[best].KQ) best = k;
+    }
+    L = gs[best].L;
+    KQ = gs[best].KQ;
+  }
+  CHK(find_kset(h, h->b, L, KQ));
+  const int w = L * KQ;
+  h->L = L;
+  h->KQ = KQ;
+  h->qp = ((qp0 + w - 1) / w) * w;
+  h->sp = h->qp * ne;
+  const int gwg = (fkd::WG / fkd::WAVE) * (fkd::WAVE / L);
+  const int cap = h->max_wg > 0 ? h->max_wg : 2048;
+  int nwg = (h->nb + gwg - 1) / gwg;
+  if (nwg > cap) nwg = cap;
+  if (nwg < 1) nwg = 1;
+  int chunk = (h->nb + nwg - 1) / nwg;
+  chunk = ((chunk + gwg - 1) / gwg) * gwg;
+  nwg = (h->nb + chunk - 1) / chunk;
+  h->nwg = nwg;
+  h->chunk = chunk;
+  h->geo_b = h->b;
+  return 0;
+}
+
+template <class TH, class S, bool CX>
+int pack_R_t(fkpm_handle* h) {
+  dim3 grid((h->n + 63) / 64, (h->sp + 15) / 16);
+  return launch(&fkd::k_pack_R<TH, S, CX>, grid, dim3(fkd::WG), h->stream,
+                h->rraw.as<const TH>(), h->R.as<S>(), h->n, h->s, h->sp);
+}
+
+template <class S>
+int pack_R_s(fkpm_handle* h) {
+  switch (h->dtype) {
+    case FKPM_F32: return pack_R_t<float, S, false>(h);
+    case FKPM_F64: return pack_R_t<double, S, false>(h);
+    case FKPM_C32: return pack_R_t<float, S, true>(h);
+    default: return pack_R_t<double, S, true>(h);
+  }
+}
+
+// Make geometry, vectors and packed R consistent with the current H and R.
+int ensure_ready(fkpm_handle* h) {
+  if (!h->h_set) return fail(FKPM_E_STATE, "set_H has not been called");
+  if (!h->r_set) return fail(FKPM_E_STATE, "R has not been set (set_R / transfer_R)");
+  if (h->n != h->nb * h->b)
+    return fail(FKPM_E_ARG, "R has %d rows but H has %d = %d blocks x %d", h->n, h->nb * h->b,
+                h->nb, h->b);
+  if (!h->r_dirty && h->geo_b == h->b) return 0;
+  CHK(choose_geometry(h));
+  const size_t vb = h->vbytes();
+  CHK(h->R.ensure(vb));
+  for (int k = 0; k < 3; k++) CHK(h->A[k].ensure(vb));
+  if (h->single)
+    CHK(pack_R_s<float>(h));
+  else
+    CHK(pack_R_s<double>(h));
+  // A-buffers: padded columns must be zero; every element is written by the
+  // recursion before it is read, so no clear is needed.
+  h->r_dirty = false;
+  h->alpha_M = 0;
+  return 0;
+}
+
+int ensure_partials(fkpm_handle* h, int nsteps) {
+  CHK(h->partial.ensure((size_t)nsteps * h->nwg * 2 * sizeof(double)));
+  CHK(h->psum.ensure((size_t)nsteps * 2 * sizeof(double)));
+  return 0;
+}
+
+int zero_doubles(fkpm_handle* h, void* p, size_t n) {
+  return launch(&fkd::k_zero, dim3(grid_for(n)), dim3(256), h->stream, (double*)p, n);
+}
+
+int timing_begin(fkpm_handle* h) {
+  if (h->timing) HIPCHK(hipEventRecord(h->ev0, h->stream));
+  return 0;
+}
+int timing_end(fkpm_handle* h, double* ms, int* cnt, int launches) {
+  if (!h->timing) return 0;
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  *ms = t;
+  *cnt = launches;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// moments (engine_cpu.cpp:63-84; engine_cusparse.cpp:755-791)
+// ---------------------------------------------------------------------------
+template <class S>
+int moments_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, double* mu) {
+  const int nsteps = M / 2;
+  CHK(ensure_partials(h, nsteps));
+  const dim3 grid(h->nwg), block(fkd::WG);
+  CHK(timing_begin(h));
+  for (int m = 0; m < nsteps; m++) {
+    // alpha_{m+1} = cx Hs alpha_m + cz alpha_{m-1}
+    const S* x = (const S*)h->vec_a(m);
+    const S* z = m == 0 ? nullptr : (const S*)h->vec_a(m - 1);
+    S* y = (S*)h->vec_a(m + 1);
+    CHK(launch(ks.mom, grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
+               h->hv.as<const S>(), x, z, y, h->nb, h->qp, h->chunk, m == 0 ? 1.0 : 2.0,
+               m == 0 ? 0.0 : -1.0, h->partial.as<double>() + (size_t)m * h->nwg * 2));
+  }
+  CHK(timing_end(h, &h->ms_mom, &h->n_mom, nsteps));
+  CHK(launch(&fkd::k_reduce_partials, dim3(nsteps), block, h->stream,
+             (const double*)h->partial.as<double>(), h->nwg, h->psum.as<double>()));
+  std::vector<double> ps((size_t)nsteps * 2);
+  HIPCHK(hipMemcpyAsync(ps.data(), h->psum.p, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  mu[0] = ps[0];
+  mu[1] = ps[1];
+  for (int m = 1; m < nsteps; m++) {
+    mu[2 * m] = 2 * ps[2 * m] - mu[0];
+    mu[2 * m + 1] = 2 * ps[2 * m + 1] - mu[1];
+  }
+  h->alpha_M = M;
+  return 0;
+}
+
+// D to the host in the engine type.
+int copy_D_out(fkpm_handle* h, void* D_val) {
+  const size_t cnt = (size_t)h->nnzbD * h->b * h->b * (h->cx ? 2 : 1);
+  if (!dtype_single(h->dtype)) {
+    HIPCHK(hipMemcpyAsync(D_val, h->dout.p, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<double> tmp(cnt);
+    HIPCHK(hipMemcpyAsync(tmp.data(), h->dout.p, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    float* o = static_cast<float*>(D_val);
+    for (size_t k = 0; k < cnt; k++) o[k] = (float)tmp[k];
+  }
+  return 0;
+}
+
+int check_D_ready(fkpm_handle* h) {
+  if (!h->d_set) return fail(FKPM_E_STATE, "D pattern has not been set");
+  if (h->nbD != h->nb)
+    return fail(FKPM_E_ARG, "D has %d block rows but H has %d", h->nbD, h->nb);
+  const size_t dbytes = (size_t)h->nnzbD * h->b * h->b * (h->cx ? 2 : 1) * sizeof(double);
+  CHK(h->dacc.ensure(dbytes > 0 ? dbytes : 16));
+  CHK(h->dout.ensure(dbytes > 0 ? dbytes : 16));
+  h->d_fused = (h->hd_rp == h->h_rp && h->hd_ci == h->h_ci);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// stoch_matrix (engine_cpu.cpp:229-253; engine_cusparse.cpp:910-953)
+// ---------------------------------------------------------------------------
+template <class S>
+int stoch_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
+  CHK(h->xi.ensure(h->accbytes()));
+  const dim3 grid(h->nwg), block(fkd::WG);
+  const int* rp = h->rp.as<const int>();
+  const int* ci = h->ci.as<const int>();
+  const S* hv = h->hv.as<const S>();
+  CHK(timing_begin(h));
+  // alpha_1 = Hs R, xi = c0 R + c1 alpha_1
+  CHK(launch(ks.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(0), (const S*)nullptr,
+             (S*)h->vec_a(1), h->xi.as<double>(), h->nb, h->qp, h->chunk, 1.0, 0.0, c[0], c[1],
+             1));
+  for (int m = 2; m < M; m++) {
+    // alpha_m = 2 Hs alpha_{m-1} - alpha_{m-2}; xi += c_m alpha_m
+    CHK(launch(ks.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(m - 1),
+               (const S*)h->vec_a(m - 2), (S*)h->vec_a(m), h->xi.as<double>(), h->nb, h->qp,
+               h->chunk, 2.0, -1.0, 0.0, c[m], 0));
+  }
+  CHK(timing_end(h, &h->ms_stoch, &h->n_stoch, M - 1));
+  const size_t dcount = (size_t)h->nnzbD * h->b * h->b * (h->cx ? 2 : 1);
+  CHK(zero_doubles(h, h->dacc.p, dcount));
+  // D = (1/2) R xi^+ + (1/2) xi R^+
+  CHK(launch(ks.outer_sd, grid, block, h->stream, h->drp.as<const int>(), h->dci.as<const int>(),
+             (const S*)h->vec_a(0), (const double*)h->xi.as<double>(), h->dacc.as<double>(), h->nb,
+             h->qp, h->chunk, 0.5));
+  CHK(launch(ks.outer_ds, grid, block, h->stream, h->drp.as<const int>(), h->dci.as<const int>(),
+             (const double*)h->xi.as<double>(), (const S*)h->vec_a(0), h->dacc.as<double>(), h->nb,
+             h->qp, h->chunk, 0.5));
+  HIPCHK(hipMemcpyAsync(h->dout.p, h->dacc.p, dcount * sizeof(double), hipMemcpyDeviceToDevice,
+                        h->stream));
+  h->alpha_M = 0;  // engine_cpu.cpp:250-252
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// autodiff_matrix (engine_cpu.cpp:255-304; engine_cusparse.cpp:955-1039)
+// ---------------------------------------------------------------------------
+template <class S>
+int autodiff_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
+  const dim3 grid(h->nwg), block(fkd::WG);
+  const int* rp = h->rp.as<const int>();
+  const int* ci = h->ci.as<const int>();
+  const S* hv = h->hv.as<const S>();
+  const int* drp = h->drp.as<const int>();
+  const int* dci = h->dci.as<const int>();
+  double* dacc = h->dacc.as<double>();
+  const size_t dcount = (size_t)h->nnzbD * h->b * h->b * (h->cx ? 2 : 1);
+
+  double diag = c[1];
+  for (int m = 1; m < M / 2; m++) diag -= c[2 * m + 1];
+  CHK(zero_doubles(h, dacc, dcount));
+  // D = diag R R^+
+  CHK(launch(ks.outer_ss, grid, block, h->stream, drp, dci, (const S*)h->vec_a(0),
+             (const S*)h->vec_a(0), dacc, h->nb, h->qp, h->chunk, diag));
+  int launches = 0;
+  CHK(timing_begin(h));
+  if (M > 2) {
+    const size_t vb = h->vbytes();
+    for (int k = 0; k < 3; k++) CHK(h->Bv[k].ensure(vb));
+    const double sb = 2 * c[M - 1];  // beta_{M/2} = 2 c_{M-1} alpha_{M/2-1}
+    for (int m = M / 2 - 1; m >= 1; m--) {
+      const S* am = (const S*)h->A[m % 3].p;       // alpha_m
+      const S* am1 = (const S*)h->A[(m + 1) % 3].p;  // alpha_{m+1}
+      S* aout = (S*)h->A[(m - 1) % 3].p;             // alpha_{m-1}
+      const bool first = (m == M / 2 - 1);
+      const bool second = (m == M / 2 - 2);
+      const S* bm1 = first ? nullptr : (const S*)h->Bv[(m + 1) % 3].p;
+      const S* bm2 = (first || second) ? nullptr : (const S*)h->Bv[(m + 2) % 3].p;
+      const double sb2 = second ? sb : 0.0;
+      S* bout = (S*)h->Bv[m % 3].p;
+      const double c4 = 4 * c[2 * m], c2 = 2 * c[2 * m + 1];
+      const double w = m > 1 ? 2 * c[2 * m - 1] : 0.0;
+      if (h->d_fused) {
+        CHK(launch(ks.ad_fused, grid, block, h->stream, rp, ci, hv, am, am1, bm1, bm2, aout, bout,
+                   dacc, h->nb, h->qp, h->chunk, c4, c2, w, sb, sb2, 2.0));
+      } else {
+        // D += 2 alpha_m beta_{m+1}^+ on D's own pattern, then the step.
+        CHK(launch(ks.outer_ss, grid, block, h->stream, drp, dci, am, first ? am : bm1, dacc,
+                   h->nb, h->qp, h->chunk, first ? 2.0 * sb : 2.0));
+        CHK(launch(ks.ad_plain, grid, block, h->stream, rp, ci, hv, am, am1, bm1, bm2, aout, bout,
+                   dacc, h->nb, h->qp, h->chunk, c4, c2, w, sb, sb2, 2.0));
+      }
+      launches++;
+    }
+    // D += alpha_0 beta_1^+
+    CHK(launch(ks.outer_ss, grid, block, h->stream, drp, dci, (const S*)h->A[0].p,
+               (const S*)h->Bv[1].p, dacc, h->nb, h->qp, h->chunk, 1.0));
+  }
+  CHK(timing_end(h, &h->ms_ad, &h->n_ad, launches));
+  // symmetrize + scale(1/mag)
+  const size_t dcnt = (size_t)h->nnzbD * h->b * h->b;
+  if (h->cx)
+    CHK(launch(&fkd::k_symmetrize_scale<true>, dim3(grid_for(dcnt)), dim3(256), h->stream,
+               (const double*)dacc, h->dout.as<double>(), h->tpos.as<const int>(), h->nnzbD, h->b,
+               1.0 / h->mag));
+  else
+    CHK(launch(&fkd::k_symmetrize_scale<false>, dim3(grid_for(dcnt)), dim3(256), h->stream,
+               (const double*)dacc, h->dout.as<double>(), h->tpos.as<const int>(), h->nnzbD, h->b,
+               1.0 / h->mag));
+  h->alpha_M = 0;  // engine_cpu.cpp:301-303
+  return 0;
+}
+
+// Upload the unscaled (or scaled) H in the storage type S from the host type.
+template <class TH, class S, bool CX>
+int upload_H_t(fkpm_handle* h, DevBuf& dst, const void* val, double mag, double shift) {
+  const size_t cnt = (size_t)h->nnzb * h->b * h->b;
+  HIPCHK(hipMemcpyAsync(h->hraw.p, val, cnt * sizeof(TH) * (CX ? 2 : 1), hipMemcpyHostToDevice,
+                        h->stream));
+  return launch(&fkd::k_scale_H<TH, S, CX>, dim3(grid_for(cnt)), dim3(256), h->stream,
+                h->hraw.as<const TH>(), dst.as<S>(), h->rowblk.as<const int>(),
+                h->ci.as<const int>(), h->nnzb, h->b, mag, shift);
+}
+
+template <class S>
+int upload_H_s(fkpm_handle* h, DevBuf& dst, const void* val, double mag, double shift) {
+  switch (h->dtype) {
+    case FKPM_F32: return upload_H_t<float, S, false>(h, dst, val, mag, shift);
+    case FKPM_F64: return upload_H_t<double, S, false>(h, dst, val, mag, shift);
+    case FKPM_C32: return upload_H_t<float, S, true>(h, dst, val, mag, shift);
+    default: return upload_H_t<double, S, true>(h, dst, val, mag, shift);
+  }
+}
+
+// Pattern upload shared by set_H and energy_scale.
+int upload_pattern(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr,
+                   const int* col_idx) {
+  h->nb = nb;
+  h->b = b;
+  h->nnzb = nnzb;
+  h->h_rp.assign(row_ptr, row_ptr + nb + 1);
+  h->h_ci.assign(col_idx, col_idx + nnzb);
+  std::vector<int> rowblk(nnzb > 0 ? nnzb : 1);
+  for (int i = 0; i < nb; i++)
+    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) rowblk[p] = i;
+  CHK(h->rp.ensure(sizeof(int) * (nb + 1)));
+  CHK(h->ci.ensure(sizeof(int) * (nnzb > 0 ? nnzb : 1)));
+  CHK(h->rowblk.ensure(sizeof(int) * rowblk.size()));
+  HIPCHK(hipMemcpyAsync(h->rp.p, row_ptr, sizeof(int) * (nb + 1), hipMemcpyHostToDevice, h->stream));
+  if (nnzb > 0)
+    HIPCHK(hipMemcpyAsync(h->ci.p, col_idx, sizeof(int) * nnzb, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->rowblk.p, rowblk.data(), sizeof(int) * rowblk.size(),
+                        hipMemcpyHostToDevice, h->stream));
+  const size_t hvb = (size_t)nnzb * b * b * host_elem_bytes(h->dtype);
+  CHK(h->hraw.ensure(hvb > 0 ? hvb : 16));
+  // the async copies above read host arrays owned by the caller: finish them
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// Symmetric tridiagonal extreme eigenvalues by Sturm-sequence bisection
+// (replaces arma::eig_sym on the Lanczos matrix, engine_cusparse.cpp:655-666).
+int sturm_below(const std::vector<double>& a, const std::vector<double>& b2, double x) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (q < 0) cnt++;
+  for (size_t j = 1; j < a.size(); j++) {
+    if (q == 0) q = 1e-300;
+    q = a[j] - x - b2[j] / q;
+    if (q < 0) cnt++;
+  }
+  return cnt;
+}
+
+void tridiag_extremes(const std::vector<double>& a, const std::vector<double>& beta,
+                      double* emin, double* emax) {
+  const size_t n = a.size();
+  std::vector<double> b2(n, 0.0);
+  double lo = a[0], hi = a[0];
+  for (size_t j = 1; j < n; j++) b2[j] = beta[j] * beta[j];
+  for (size_t j = 0; j < n; j++) {
+    const double r = (j > 0 ? std::fabs(beta[j]) : 0.0) + (j + 1 < n ? std::fabs(beta[j + 1]) : 0.0);
+    lo = std::min(lo, a[j] - r);
+    hi = std::max(hi, a[j] + r);
+  }
+  const double pad = 1e-12 * (std::fabs(lo) + std::fabs(hi) + 1.0);
+  auto kth = [&](int k) {
+    double l = lo - pad, u = hi + pad;
+    for (int it = 0; it < 200; it++) {
+      const double mid = 0.5 * (l + u);
+      if (mid == l || mid == u) break;
+      if (sturm_below(a, b2, mid) > k) u = mid; else l = mid;
+    }
+    return 0.5 * (l + u);
+  };
+  *emin = kth(0);
+  *emax = kth((int)n - 1);
+}
+
+// Lanczos on the device in FP64 (engine_cusparse.cpp:598-673).  Uses its own
+// buffers, so the engine's H is untouched.  A diagonal block missing from H is
+// inserted as an explicit zero block: the fused SpMM kernel takes the own-row
+// operand of the dot from the diagonal block.
+template <class TH, bool CX>
+int lanczos_t(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const int* col_idx,
+              const void* val, double extend, int iters, const void* v_start, double* lo,
+              double* hi) {
+  fkd::KSet<double> ks;
+  const int geoL = 4, geoKQ = 1;
+  const bool ok = CX ? fkd::find_kset_dc(b, geoL, geoKQ, &ks) : fkd::find_kset_dr(b, geoL, geoKQ, &ks);
+  if (!ok) return fail(FKPM_E_UNSUP, "no Lanczos kernel for block %d", b);
+  const int HS = CX ? 2 : 1;
+  const int bb = b * b;
+  // pattern with every diagonal block present
+  std::vector<int> rp2(nb + 1), ci2;
+  std::vector<double> hv2;
+  ci2.reserve((size_t)nnzb + nb);
+  hv2.reserve(((size_t)nnzb + nb) * bb * HS);
+  rp2[0] = 0;
+  for (int i = 0; i < nb; i++) {
+    bool diag = false;
+    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) diag |= (col_idx[p] == i);
+    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) {
+      ci2.push_back(col_idx[p]);
+      const TH* v = static_cast<const TH*>(val) + (size_t)p * bb * HS;
+      for (int e = 0; e < bb * HS; e++) hv2.push_back((double)v[e]);
+    }
+    if (!diag) {
+      ci2.push_back(i);
+      for (int e = 0; e < bb * HS; e++) hv2.push_back(0.0);
+    }
+    rp2[i + 1] = (int)ci2.size();
+  }
+  const int nnz2 = (int)ci2.size();
+  const int n = nb * b;
+  const int qp = geoL * geoKQ;  // one probe column padded to one lane-group pass
+  const int sp = qp * (2 / HS);
+  const size_t len = (size_t)n * sp * HS;  // doubles per vector
+  const int gwg = (fkd::WG / fkd::WAVE) * (fkd::WAVE / geoL);
+  int nwg = std::min((nb + gwg - 1) / gwg, 1024);
+  int chunk = (nb + nwg - 1) / nwg;
+  chunk = ((chunk + gwg - 1) / gwg) * gwg;
+  nwg = (nb + chunk - 1) / chunk;
+
+  struct Bufs {
+    DevBuf rp, ci, hv, v0, v1, w, part, ps;
+    ~Bufs() {
+      DevBuf* all[] = {&rp, &ci, &hv, &v0, &v1, &w, &part, &ps};
+      for (DevBuf* d : all) d->release();
+    }
+  } B;
+  CHK(B.rp.ensure(sizeof(int) * rp2.size()));
+  CHK(B.ci.ensure(sizeof(int) * ci2.size()));
+  CHK(B.hv.ensure(sizeof(double) * hv2.size()));
+  CHK(B.v0.ensure(len * sizeof(double)));
+  CHK(B.v1.ensure(len * sizeof(double)));
+  CHK(B.w.ensure(len * sizeof(double)));
+  CHK(B.part.ensure((size_t)nwg * 2 * sizeof(double)));
+  CHK(B.ps.ensure(2 * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(B.rp.p, rp2.data(), sizeof(int) * rp2.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(B.ci.p, ci2.data(), sizeof(int) * ci2.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(B.hv.p, hv2.data(), sizeof(double) * hv2.size(), hipMemcpyHostToDevice, h->stream));
+  CHK(zero_doubles(h, B.v0.p, len));
+  CHK(zero_doubles(h, B.w.p, len));
+  {
+    std::vector<double> packed(len, 0.0);
+    double nrm = 0.0;
+    for (int r = 0; r < n; r++)
+      for (int c = 0; c < HS; c++) {
+        const double x = (double)static_cast<const TH*>(v_start)[(size_t)r * HS + c];
+        nrm += x * x;
+      }
+    nrm = std::sqrt(nrm);
+    if (!(nrm > 0) || !std::isfinite(nrm)) return fail(FKPM_E_ARG, "energy_scale: bad start vector");
+    for (int r = 0; r < n; r++)
+      for (int c = 0; c < HS; c++)
+        packed[(size_t)r * sp * HS + c] = (double)static_cast<const TH*>(v_start)[(size_t)r * HS + c] / nrm;
+    HIPCHK(hipMemcpyAsync(B.v1.p, packed.data(), len * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));  // host staging vectors go out of scope
+  }
+  const dim3 grid(nwg), block(fkd::WG);
+  double ps[2];
+  auto spmv_dot = [&](double* out_alpha) -> int {
+    // w = H v1, alpha = Re<w, v1>
+    CHK(launch(ks.mom, grid, block, h->stream, (const int*)B.rp.p, (const int*)B.ci.p,
+               (const double*)B.hv.p, (const double*)B.v1.p, (const double*)nullptr,
+               (double*)B.w.p, nb, qp, chunk, 1.0, 0.0, (double*)B.part.p));
+    CHK(launch(&fkd::k_reduce_partials, dim3(1), block, h->stream, (const double*)B.part.p, nwg,
+               (double*)B.ps.p));
+    HIPCHK(hipMemcpyAsync(ps, B.ps.p, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *out_alpha = ps[1];
+    return 0;
+  };
+  std::vector<double> alpha(iters, 0.0), beta(iters, 0.0);
+  int used = iters;
+  for (int j = 1; j < iters; j++) {
+    CHK(spmv_dot(&alpha[j - 1]));
+    // w = w - alpha v1 - beta v0; beta_j = |w|
+    CHK(launch(&fkd::k_lanczos_orth, dim3(nwg), block, h->stream, (double*)B.w.p,
+               (const double*)B.v1.p, (const double*)B.v0.p, len, alpha[j - 1], beta[j - 1],
+               (double*)B.part.p));
+    CHK(launch(&fkd::k_reduce_partials, dim3(1), block, h->stream, (const double*)B.part.p, nwg,
+               (double*)B.ps.p));
+    HIPCHK(hipMemcpyAsync(ps, B.ps.p, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    beta[j] = std::sqrt(ps[0]);
+    if (!(beta[j] > 1e-300)) {  // invariant subspace: the j x j matrix is exact
+      used = j;
+      break;
+    }
+    CHK(launch(&fkd::k_lanczos_next, dim3(grid_for(len)), dim3(256), h->stream, (double*)B.v0.p,
+               (double*)B.v1.p, (const double*)B.w.p, len, 1.0 / beta[j]));
+  }
+  if (used == iters) CHK(spmv_dot(&alpha[iters - 1]));
+  alpha.resize(used);
+  beta.resize(used);
+  double emin = 0, emax = 0;
+  tridiag_extremes(alpha, beta, &emin, &emax);
+  const double slack = extend * (emax - emin);
+  *lo = emin - slack;
+  *hi = emax + slack;
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* fkpm_last_error(void) { return g_last_error.c_str(); }
+
+int fkpm_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int fkpm_dev_open(int device, int dtype, int prec, fkpm_handle** out) {
+  if (!out) return fail(FKPM_E_ARG, "null handle pointer");
+  *out = nullptr;
+  if (dtype < FKPM_F32 || dtype > FKPM_C64) return fail(FKPM_E_ARG, "bad dtype %d", dtype);
+  if (prec != FKPM_PREC_FP64 && prec != FKPM_PREC_FP32) return fail(FKPM_E_ARG, "bad prec %d", prec);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(FKPM_E_NODEV, "no HIP device is available");
+  if (device < 0 || device >= count)
+    return fail(FKPM_E_NODEV, "invalid device #%d (%d available)", device, count);
+  HIPCHK(hipSetDevice(device));
+  fkpm_handle* h = new fkpm_handle();
+  h->device = device;
+  h->dtype = dtype;
+  h->prec = prec;
+  h->cx = dtype_complex(dtype);
+  h->single = (prec == FKPM_PREC_FP32);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+    delete h;
+    return fail(FKPM_E_HIP, "stream/event creation failed");
+  }
+  *out = h;
+  return 0;
+}
+
+int fkpm_dev_close(fkpm_handle* h) {
+  if (!h) return 0;
+  delete h;
+  return 0;
+}
+
+void* fkpm_stream(fkpm_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+int fkpm_set_H(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const int* col_idx,
+               const void* val, double avg, double mag) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  if (b <= 0) return fail(FKPM_E_ARG, "bad block size %d", b);
+  if (!fkd::block_supported(b)) return fail(FKPM_E_UNSUP, "block size %d has no kernels", b);
+  if (!(mag > 0) || !std::isfinite(avg)) return fail(FKPM_E_ARG, "bad energy scale avg=%g mag=%g", avg, mag);
+  if (nnzb > 0 && !val) return fail(FKPM_E_ARG, "null H values");
+  CHK(check_pattern("set_H", nb, nnzb, row_ptr, nullptr, col_idx));
+  for (int i = 0; i < nb; i++) {  // engine_cusparse.cpp:716-727: every diagonal block present
+    bool found = false;
+    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) found |= (col_idx[p] == i);
+    if (!found) return fail(FKPM_E_ARG, "set_H: diagonal block (%d,%d) is missing", i, i);
+  }
+  const bool geo_change = (b != h->b);
+  CHK(upload_pattern(h, nb, b, nnzb, row_ptr, col_idx));
+  h->avg = avg;
+  h->mag = mag;
+  const size_t eb = h->single ? 4 : 8;
+  CHK(h->hv.ensure((size_t)(nnzb > 0 ? nnzb : 1) * b * b * (h->cx ? 2 : 1) * eb));
+  if (h->single)
+    CHK(upload_H_s<float>(h, h->hv, val, mag, avg / mag));
+  else
+    CHK(upload_H_s<double>(h, h->hv, val, mag, avg / mag));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->h_set = true;
+  h->alpha_M = 0;
+  if (geo_change) h->r_dirty = true;
+  return 0;
+}
+
+int fkpm_set_R(fkpm_handle* h, int n, int s, const void* R_colmajor) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  if (n <= 0 || s <= 0 || !R_colmajor)  // engine_cusparse.cpp:678-681
+    return fail(FKPM_E_ARG, "cannot handle an empty R matrix (n=%d, s=%d)", n, s);
+  const size_t bytes = (size_t)n * s * host_elem_bytes(h->dtype);
+  CHK(h->rraw.ensure(bytes));
+  HIPCHK(hipMemcpyAsync(h->rraw.p, R_colmajor, bytes, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->n = n;
+  h->s = s;
+  h->r_set = true;
+  h->r_dirty = true;
+  h->alpha_M = 0;
+  return 0;
+}
+
+int fkpm_moments(fkpm_handle* h, int M, double* mu) {
+  if (!h || !mu) return fail(FKPM_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  if (M < 2 || M % 2 != 0) return fail(FKPM_E_ARG, "moments: M=%d must be even and >= 2", M);
+  CHK(ensure_ready(h));
+  h->alpha_M = 0;
+  if (h->single) return moments_t<float>(h, h->kf, M, mu);
+  return moments_t<double>(h, h->kd, M, mu);
+}
+
+int fkpm_set_D_pattern(fkpm_handle* h, int nbD, int nnzbD, const int* row_ptr, const int* row_idx,
+                       const int* col_idx) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  CHK(check_pattern("D pattern", nbD, nnzbD, row_ptr, row_idx, col_idx));
+  // block (i,j) -> index of block (j,i), or -1
+  std::vector<int> tpos(nnzbD > 0 ? nnzbD : 1, -1);
+  bool sym = true;
+  for (int i = 0; i < nbD; i++)
+    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) {
+      const int j = col_idx[p];
+      int t = -1;
+      for (int q = row_ptr[j]; q < row_ptr[j + 1]; q++)
+        if (col_idx[q] == i) {
+          t = q;
+          break;
+        }
+      tpos[p] = t;
+      if (t < 0) sym = false;
+    }
+  CHK(h->drp.ensure(sizeof(int) * (nbD + 1)));
+  CHK(h->dci.ensure(sizeof(int) * (nnzbD > 0 ? nnzbD : 1)));
+  CHK(h->tpos.ensure(sizeof(int) * tpos.size()));
+  HIPCHK(hipMemcpyAsync(h->drp.p, row_ptr, sizeof(int) * (nbD + 1), hipMemcpyHostToDevice, h->stream));
+  if (nnzbD > 0)
+    HIPCHK(hipMemcpyAsync(h->dci.p, col_idx, sizeof(int) * nnzbD, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->tpos.p, tpos.data(), sizeof(int) * tpos.size(), hipMemcpyHostToDevice,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->nbD = nbD;
+  h->nnzbD = nnzbD;
+  h->hd_rp.assign(row_ptr, row_ptr + nbD + 1);
+  h->hd_ci.assign(col_idx, col_idx + nnzbD);
+  h->d_set = true;
+  h->d_sym = sym;
+  h->d_valid = false;
+  return 0;
+}
+
+int fkpm_stoch_matrix(fkpm_handle* h, int M, const double* c, void* D_val) {
+  if (!h || !c) return fail(FKPM_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  if (M < 2) return fail(FKPM_E_ARG, "stoch_matrix: need at least 2 coefficients (M=%d)", M);
+  CHK(ensure_ready(h));
+  CHK(check_D_ready(h));
+  h->d_valid = false;
+  if (h->single)
+    CHK(stoch_t<float>(h, h->kf, M, c));
+  else
+    CHK(stoch_t<double>(h, h->kd, M, c));
+  h->d_valid = true;
+  if (D_val) return copy_D_out(h, D_val);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fkpm_autodiff_matrix(fkpm_handle* h, int M, const double* c, void* D_val) {
+  if (!h || !c) return fail(FKPM_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  if (M < 2 || M % 2 != 0) return fail(FKPM_E_ARG, "autodiff_matrix: M=%d must be even and >= 2", M);
+  if (h->alpha_M != M)
+    return fail(FKPM_E_STATE,
+                "autodiff_matrix requires moments(M) with the same M (%d) immediately before "
+                "(held: %d)", M, h->alpha_M);
+  CHK(check_D_ready(h));
+  if (!h->d_sym)
+    return fail(FKPM_E_ARG, "autodiff_matrix: D pattern is not structurally symmetric");
+  h->d_valid = false;
+  if (h->single)
+    CHK(autodiff_t<float>(h, h->kf, M, c));
+  else
+    CHK(autodiff_t<double>(h, h->kd, M, c));
+  h->d_valid = true;
+  if (D_val) return copy_D_out(h, D_val);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fkpm_get_D(fkpm_handle* h, void* D_val) {
+  if (!h || !D_val) return fail(FKPM_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  if (!h->d_valid) return fail(FKPM_E_STATE, "no D has been computed");
+  return copy_D_out(h, D_val);
+}
+
+int fkpm_energy_scale(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr,
+                      const int* col_idx, const void* val, double extend, int iters,
+                      const void* v_start, double* lo, double* hi) {
+  if (!h || !v_start || !lo || !hi) return fail(FKPM_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  if (b <= 0 || !fkd::block_supported(b)) return fail(FKPM_E_UNSUP, "block size %d has no kernels", b);
+  if (iters < 1) return fail(FKPM_E_ARG, "energy_scale: iters=%d", iters);
+  if (nnzb > 0 && !val) return fail(FKPM_E_ARG, "null H values");
+  CHK(check_pattern("energy_scale", nb, nnzb, row_ptr, nullptr, col_idx));
+  switch (h->dtype) {
+    case FKPM_F32: return lanczos_t<float, false>(h, nb, b, nnzb, row_ptr, col_idx, val, extend, iters, v_start, lo, hi);
+    case FKPM_F64: return lanczos_t<double, false>(h, nb, b, nnzb, row_ptr, col_idx, val, extend, iters, v_start, lo, hi);
+    case FKPM_C32: return lanczos_t<float, true>(h, nb, b, nnzb, row_ptr, col_idx, val, extend, iters, v_start, lo, hi);
+    default: return lanczos_t<double, true>(h, nb, b, nnzb, row_ptr, col_idx, val, extend, iters, v_start, lo, hi);
+  }
+}
+
+int fkpm_set_timing(fkpm_handle* h, int on) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  h->timing = on != 0;
+  return 0;
+}
+
+int fkpm_get_timing(fkpm_handle* h, double* ms_moments, int* n_moments, double* ms_stoch,
+                    int* n_stoch, double* ms_autodiff, int* n_autodiff) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  if (ms_moments) *ms_moments = h->ms_mom;
+  if (n_moments) *n_moments = h->n_mom;
+  if (ms_stoch) *ms_stoch = h->ms_stoch;
+  if (n_stoch) *n_stoch = h->n_stoch;
+  if (ms_autodiff) *ms_autodiff = h->ms_ad;
+  if (n_autodiff) *n_autodiff = h->n_ad;
+  return 0;
+}
+
+int fkpm_get_config(fkpm_handle* h, int* lanes, int* kq, int* s_padded, int* n_wg,
+                    int* fused_autodiff) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  if (lanes) *lanes = h->L;
+  if (kq) *kq = h->KQ;
+  if (s_padded) *s_padded = h->sp;
+  if (n_wg) *n_wg = h->nwg;
+  if (fused_autodiff) *fused_autodiff = h->d_fused ? 1 : 0;
+  return 0;
+}
+
+int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  if (lanes != 0) {
+    fkd::GeomChoice gs[32];
+    const int ng = fkd::geometries_for_block(h->b > 0 ? h->b : 1, gs, 32);
+    bool ok = false;
+    for (int k = 0; k < ng; k++) ok |= (gs[k].L == lanes && gs[k].KQ == (kq > 0 ? kq : 1));
+    if (h->b > 0 && !ok) return fail(FKPM_E_UNSUP, "geometry lanes=%d kq=%d not compiled for block %d", lanes, kq, h->b);
+  }
+  h->force_L = lanes;
+  h->force_KQ = kq;
+  h->max_wg = max_wg;
+  h->r_dirty = true;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace fkd {
+int geometries_for_block(int b, GeomChoice* out, int cap) {
+  int n = 0;
+#define FKD_GEO(bb, l, kq) \
+  if (bb == b && n < cap) out[n++] = GeomChoice{l, kq};
+  FKD_FOR_EACH_GEOMETRY(FKD_GEO)
+#undef FKD_GEO
+  return n;
+}
+}  // namespace fkd

@@ -1,0 +1,55 @@
+// kpm_dispatch.h -- host-side table of kernel instantiations.
+//
+// One KSet holds the host stubs of every kernel for one (storage type S,
+// real/complex, block size B, lanes per group L, packets per lane KQ).  The
+// instantiations live in four translation units (kpm_inst_{d,f}{c,r}.hip) that
+// compile in parallel; kpm_capi.hip looks sets up by runtime parameters.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fkd {
+
+template <class S>
+struct KSet {
+  void (*mom)(const int*, const int*, const S*, const S*, const S*, S*, int, int, int, double,
+              double, double*) = nullptr;
+  void (*xi)(const int*, const int*, const S*, const S*, const S*, S*, double*, int, int, int,
+             double, double, double, double, int) = nullptr;
+  void (*ad_fused)(const int*, const int*, const S*, const S*, const S*, const S*, const S*, S*,
+                   S*, double*, int, int, int, double, double, double, double, double,
+                   double) = nullptr;
+  void (*ad_plain)(const int*, const int*, const S*, const S*, const S*, const S*, const S*, S*,
+                   S*, double*, int, int, int, double, double, double, double, double,
+                   double) = nullptr;
+  void (*outer_ss)(const int*, const int*, const S*, const S*, double*, int, int, int,
+                   double) = nullptr;
+  void (*outer_sd)(const int*, const int*, const S*, const double*, double*, int, int, int,
+                   double) = nullptr;
+  void (*outer_ds)(const int*, const int*, const double*, const S*, double*, int, int, int,
+                   double) = nullptr;
+  int B = 0, L = 0, KQ = 0;
+};
+
+// Geometries compiled for each block size (lanes per group, packets per lane).
+struct GeomChoice {
+  int L, KQ;
+};
+// Block sizes with compiled kernels.
+inline bool block_supported(int b) { return b == 1 || b == 2 || b == 4 || b == 8; }
+// List of (L, KQ) compiled for block size b; returns the count.
+int geometries_for_block(int b, GeomChoice* out, int cap);
+
+bool find_kset_dc(int B, int L, int KQ, KSet<double>* out);  // double, complex
+bool find_kset_dr(int B, int L, int KQ, KSet<double>* out);  // double, real
+bool find_kset_fc(int B, int L, int KQ, KSet<float>* out);   // float, complex
+bool find_kset_fr(int B, int L, int KQ, KSet<float>* out);   // float, real
+
+}  // namespace fkd
+
+// The list of compiled (B, L, KQ) geometries.  B = 1, 2 take up to 4 packets
+// per lane; B = 4 up to 2; B = 8 one (its accumulators are 8x larger).
+#define FKD_FOR_EACH_GEOMETRY(X) \
+  X(1, 4, 1) X(1, 4, 2) X(1, 8, 2) X(1, 16, 2) X(1, 16, 4) X(1, 32, 4) X(1, 64, 1) X(1, 64, 4) \
+  X(2, 4, 1) X(2, 4, 2) X(2, 8, 2) X(2, 16, 2) X(2, 16, 4) X(2, 32, 4) X(2, 64, 1) X(2, 64, 4) \
+  X(4, 4, 1) X(4, 4, 2) X(4, 8, 2) X(4, 16, 2) X(4, 32, 2) X(4, 64, 1) X(4, 64, 2)             \
+  X(8, 4, 1) X(8, 8, 1) X(8, 16, 1) X(8, 32, 1) X(8, 64, 1)

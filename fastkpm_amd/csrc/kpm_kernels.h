@@ -1,0 +1,759 @@
+// kpm_kernels.h -- gfx950 kernels of the stochastic Chebyshev engine.
+//
+// Layout in HBM (all kernels):
+//   Hs        BSR: row_ptr[nb+1], col_idx[nnzb] (int32), val[nnzb*B*B] in the
+//             storage type S (complex interleaved), blocks column-major.
+//   vectors   alpha_m, beta_m, R: N x Sp row-major (the Sp probe columns of one
+//             orbital contiguous), Sp = S padded so that a row is a whole
+//             number of 16-byte packets per lane group; padded columns are 0
+//             and stay 0 under the recursion.
+//   xi, D     FP64 accumulators (complex interleaved when CX).
+//
+// Work decomposition: a wavefront is split into 64/L groups of L lanes; a
+// group owns one block row i (B orbital rows) and sweeps its Sp columns in
+// passes of L*KQ packets, each lane holding KQ 16-byte packets per orbital row.
+// Workgroups take contiguous ranges of block rows, and the workgroup index is
+// remapped so that each of the 8 XCDs (which receive workgroups round-robin)
+// owns one contiguous eighth of the rows: the gathers of a neighbour row then
+// hit that XCD's L2.
+//
+// Reductions (moment dots, D blocks) are deterministic: fixed lane butterflies,
+// fixed LDS order, per-workgroup partials reduced in index order by a second
+// kernel.  No floating-point atomics anywhere.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fkd {
+
+constexpr int WAVE = 64;
+constexpr int WG = 256;
+
+template <typename T> struct Vec16;
+template <> struct Vec16<double> {
+  typedef double V __attribute__((ext_vector_type(2)));
+  static constexpr int N = 2;
+};
+template <> struct Vec16<float> {
+  typedef float V __attribute__((ext_vector_type(4)));
+  static constexpr int N = 4;
+};
+
+// Load NSC scalars of type T (a multiple of one 16-byte vector) into doubles.
+template <typename T, int NSC>
+__device__ __forceinline__ void ldd(const T* __restrict__ p, double* v) {
+  typedef typename Vec16<T>::V VT;
+  constexpr int PER = Vec16<T>::N;
+  static_assert(NSC % PER == 0, "load width");
+#pragma unroll
+  for (int c = 0; c < NSC / PER; c++) {
+    VT t = *reinterpret_cast<const VT*>(p + c * PER);
+#pragma unroll
+    for (int e = 0; e < PER; e++) v[c * PER + e] = (double)t[e];
+  }
+}
+
+// Store NSC doubles as scalars of type T.
+template <typename T, int NSC>
+__device__ __forceinline__ void std_(T* __restrict__ p, const double* v) {
+  typedef typename Vec16<T>::V VT;
+  constexpr int PER = Vec16<T>::N;
+  static_assert(NSC % PER == 0, "store width");
+#pragma unroll
+  for (int c = 0; c < NSC / PER; c++) {
+    VT t;
+#pragma unroll
+    for (int e = 0; e < PER; e++) t[e] = (T)v[c * PER + e];
+    *reinterpret_cast<VT*>(p + c * PER) = t;
+  }
+}
+
+// Round doubles to the storage type and back (the value the next step reads).
+template <typename T>
+__device__ __forceinline__ double rnd(double x) { return (double)(T)x; }
+
+// Workgroup index remapped so that XCD x (blockIdx % 8 == x) owns a
+// contiguous range of logical workgroups.
+__device__ __forceinline__ int logical_wg() {
+  const int k = blockIdx.x, n = gridDim.x;
+  const int x = k & 7, r = k >> 3;
+  const int per = n >> 3, extra = n & 7;
+  return x * per + (x < extra ? x : extra) + r;
+}
+
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int m = L / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) { return group_sum<WAVE>(v); }
+
+// Sum two per-thread doubles over the workgroup; thread 0 writes them.
+__device__ __forceinline__ void wg_sum2(double a, double b, double* out) {
+  __shared__ double sm[2][WG / WAVE];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    sm[0][w] = a;
+    sm[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int k = 0; k < WG / WAVE; k++) {
+      sa += sm[0][k];
+      sb += sm[1][k];
+    }
+    out[0] = sa;
+    out[1] = sb;
+  }
+}
+
+// Complex (or real) multiply-accumulate on interleaved scalars:
+// acc[e] += h * x[e] for the NE elements of a packet.
+template <bool CX, int NSC>
+__device__ __forceinline__ void cmac(double* acc, double hr, double hi, const double* x) {
+  if constexpr (CX) {
+#pragma unroll
+    for (int e = 0; e < NSC; e += 2) {
+      acc[e] = fma(hr, x[e], acc[e]);
+      acc[e] = fma(-hi, x[e + 1], acc[e]);
+      acc[e + 1] = fma(hr, x[e + 1], acc[e + 1]);
+      acc[e + 1] = fma(hi, x[e], acc[e + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < NSC; e++) acc[e] = fma(hr, x[e], acc[e]);
+  }
+}
+
+// Static geometry of one kernel instantiation.
+template <typename S, bool CX, int B, int L, int KQ>
+struct Geo {
+  static constexpr int NS = Vec16<S>::N;        // scalars per packet
+  static constexpr int HS = CX ? 2 : 1;          // scalars per element
+  static constexpr int NE = NS / HS;             // elements (columns) per packet
+  static constexpr int G = WAVE / L;             // groups per wave
+  static constexpr int GWG = (WG / WAVE) * G;    // groups (block rows) per workgroup pass
+  static constexpr int BBS = B * B * HS;         // scalars per H block
+  static constexpr int DE = B * B;               // elements per D block
+  static constexpr int DOWN = (DE + L - 1) / L;  // D elements owned per lane
+};
+
+struct RowRange {
+  int r0, r1;
+};
+
+__device__ __forceinline__ RowRange wg_rows(int nb, int chunk) {
+  const int lw = logical_wg();
+  RowRange r;
+  r.r0 = lw * chunk;
+  r.r1 = min(nb, r.r0 + chunk);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Moments step: y = cx * (Hs x) + cz * z  (z may be null), and the two moment
+// dots sum |x_i|^2 and Re<y_i, x_i> over this workgroup's rows (i = own rows).
+// This is the forward recursion alpha_{m+1} = 2 Hs alpha_m - alpha_{m-1} of
+// Engine_CPU::moments (engine_cpu.cpp:63-84) fused with both cdot's.
+// ---------------------------------------------------------------------------
+template <typename S, bool CX, int B, int L, int KQ>
+__global__ __launch_bounds__(WG) void k_step_moments(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial) {
+  using GE = Geo<S, CX, B, L, KQ>;
+  constexpr int NS = GE::NS;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = lane % L;
+  const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
+  const RowRange rr = wg_rows(nb, chunk);
+  const size_t rs = (size_t)qp * NS;  // scalars per orbital row
+  double dmm = 0.0, dm1 = 0.0;
+
+  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+    const int i = base + g;
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+      double acc[KQ][B][NS];
+      double xd[KQ][B][NS];
+#pragma unroll
+      for (int k = 0; k < KQ; k++)
+#pragma unroll
+        for (int bi = 0; bi < B; bi++)
+#pragma unroll
+          for (int e = 0; e < NS; e++) {
+            acc[k][bi][e] = 0.0;
+            xd[k][bi][e] = 0.0;
+          }
+      for (int p = p0; p < p1; p++) {
+        const int j = ci[p];
+        const S* hb = hv + (size_t)p * GE::BBS;
+        const bool diag = (j == i);
+#pragma unroll
+        for (int bj = 0; bj < B; bj++) {
+          double hc[B * GE::HS];  // block column bj (column-major block)
+#pragma unroll
+          for (int t = 0; t < B * GE::HS; t++) hc[t] = (double)hb[bj * B * GE::HS + t];
+          const S* xr = x + (size_t)(B * j + bj) * rs;
+#pragma unroll
+          for (int k = 0; k < KQ; k++) {
+            double xv[NS];
+            ldd<S, NS>(xr + (size_t)(q0 + k * L + l) * NS, xv);
+            if (diag) {
+#pragma unroll
+              for (int e = 0; e < NS; e++) xd[k][bj][e] = xv[e];
+            }
+#pragma unroll
+            for (int bi = 0; bi < B; bi++)
+              cmac<CX, NS>(acc[k][bi], hc[bi * GE::HS], CX ? hc[bi * GE::HS + GE::HS - 1] : 0.0, xv);
+          }
+        }
+      }
+#pragma unroll
+      for (int bi = 0; bi < B; bi++) {
+        const size_t ro = (size_t)(B * i + bi) * rs;
+#pragma unroll
+        for (int k = 0; k < KQ; k++) {
+          const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
+          double zv[NS];
+          if (z != nullptr) {
+            ldd<S, NS>(z + off, zv);
+          } else {
+#pragma unroll
+            for (int e = 0; e < NS; e++) zv[e] = 0.0;
+          }
+          double out[NS];
+#pragma unroll
+          for (int e = 0; e < NS; e++) out[e] = rnd<S>(fma(cx, acc[k][bi][e], cz * zv[e]));
+          std_<S, NS>(y + off, out);
+#pragma unroll
+          for (int e = 0; e < NS; e++) {
+            dmm = fma(xd[k][bi][e], xd[k][bi][e], dmm);
+            dm1 = fma(out[e], xd[k][bi][e], dm1);
+          }
+        }
+      }
+    }
+  }
+  wg_sum2(dmm, dm1, partial + 2 * (size_t)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Stochastic-matrix step: y = cx * (Hs x) + cz * z, and xi (FP64) updated as
+// xi = c0 * x_i + c1 * y  (first step, x = R) or xi += c1 * y.  This is the
+// loop body of Engine_CPU::stoch_matrix (engine_cpu.cpp:229-244).
+// ---------------------------------------------------------------------------
+template <typename S, bool CX, int B, int L, int KQ>
+__global__ __launch_bounds__(WG) void k_step_xi(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y,
+    double* __restrict__ xi, int nb, int qp, int chunk, double cx, double cz, double c0,
+    double c1, int first) {
+  using GE = Geo<S, CX, B, L, KQ>;
+  constexpr int NS = GE::NS;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = lane % L;
+  const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
+  const RowRange rr = wg_rows(nb, chunk);
+  const size_t rs = (size_t)qp * NS;
+
+  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+    const int i = base + g;
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+      double acc[KQ][B][NS];
+      double xd[KQ][B][NS];
+#pragma unroll
+      for (int k = 0; k < KQ; k++)
+#pragma unroll
+        for (int bi = 0; bi < B; bi++)
+#pragma unroll
+          for (int e = 0; e < NS; e++) {
+            acc[k][bi][e] = 0.0;
+            xd[k][bi][e] = 0.0;
+          }
+      for (int p = p0; p < p1; p++) {
+        const int j = ci[p];
+        const S* hb = hv + (size_t)p * GE::BBS;
+        const bool diag = (j == i);
+#pragma unroll
+        for (int bj = 0; bj < B; bj++) {
+          double hc[B * GE::HS];  // block column bj (column-major block)
+#pragma unroll
+          for (int t = 0; t < B * GE::HS; t++) hc[t] = (double)hb[bj * B * GE::HS + t];
+          const S* xr = x + (size_t)(B * j + bj) * rs;
+#pragma unroll
+          for (int k = 0; k < KQ; k++) {
+            double xv[NS];
+            ldd<S, NS>(xr + (size_t)(q0 + k * L + l) * NS, xv);
+            if (diag) {
+#pragma unroll
+              for (int e = 0; e < NS; e++) xd[k][bj][e] = xv[e];
+            }
+#pragma unroll
+            for (int bi = 0; bi < B; bi++)
+              cmac<CX, NS>(acc[k][bi], hc[bi * GE::HS], CX ? hc[bi * GE::HS + GE::HS - 1] : 0.0, xv);
+          }
+        }
+      }
+#pragma unroll
+      for (int bi = 0; bi < B; bi++) {
+        const size_t ro = (size_t)(B * i + bi) * rs;
+#pragma unroll
+        for (int k = 0; k < KQ; k++) {
+          const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
+          double zv[NS];
+          if (z != nullptr) {
+            ldd<S, NS>(z + off, zv);
+          } else {
+#pragma unroll
+            for (int e = 0; e < NS; e++) zv[e] = 0.0;
+          }
+          double out[NS];
+#pragma unroll
+          for (int e = 0; e < NS; e++) out[e] = rnd<S>(fma(cx, acc[k][bi][e], cz * zv[e]));
+          std_<S, NS>(y + off, out);
+          double xa[NS];
+          if (first) {
+#pragma unroll
+            for (int e = 0; e < NS; e++) xa[e] = fma(c0, xd[k][bi][e], c1 * out[e]);
+          } else {
+            ldd<double, NS>(xi + off, xa);
+#pragma unroll
+            for (int e = 0; e < NS; e++) xa[e] = fma(c1, out[e], xa[e]);
+          }
+          std_<double, NS>(xi + off, xa);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Autodiff backward step m (Engine_CPU::autodiff_matrix, engine_cpu.cpp:278-294):
+//   a_out = alpha_{m-1} = 2 Hs alpha_m - alpha_{m+1}
+//   b_out = beta_m      = 4 c_{2m} alpha_m + 2 c_{2m+1} alpha_{m+1}
+//                         + 2 Hs beta_{m+1} - beta_{m+2} + w alpha_{m-1}
+//   D    += 2 alpha_m beta_{m+1}^+        (FUSED_D: on H's own pattern)
+// beta_{m+1} may be virtual: bm1 == null means beta_{m+1} = sb * alpha_m
+// (the step m = M/2-1, where beta_{M/2} = 2 c_{M-1} alpha_{M/2-1}); bm2 == null
+// means beta_{m+2} = sb2 * alpha_{m+1} (sb2 = 0 at the first step).
+// ---------------------------------------------------------------------------
+template <typename S, bool CX, int B, int L, int KQ, bool FUSED_D>
+__global__ __launch_bounds__(WG) void k_step_autodiff(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale) {
+  using GE = Geo<S, CX, B, L, KQ>;
+  constexpr int NS = GE::NS;
+  constexpr int HS = GE::HS;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = lane % L;
+  const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
+  const RowRange rr = wg_rows(nb, chunk);
+  const size_t rs = (size_t)qp * NS;
+  const bool vb1 = (bm1 == nullptr);
+  const double hb1 = vb1 ? sb : 1.0;  // scale of the beta_{m+1} gather
+
+  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+    const int i = base + g;
+    const bool active = i < rr.r1;
+    // Inactive groups still run the lane butterflies of FUSED_D below with
+    // zero rows (the shuffles need every lane of the wave), but load nothing.
+    const int p0 = active ? rp[i] : 0;
+    const int p1 = active ? rp[i + 1] : 0;
+    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+      double ai[KQ][B][NS];
+      double acca[KQ][B][NS];
+      double accb[KQ][B][NS];
+#pragma unroll
+      for (int bi = 0; bi < B; bi++) {
+        const size_t ro = (size_t)(B * (active ? i : 0) + bi) * rs;
+#pragma unroll
+        for (int k = 0; k < KQ; k++) {
+          if (active) {
+            ldd<S, NS>(am + ro + (size_t)(q0 + k * L + l) * NS, ai[k][bi]);
+          } else {
+#pragma unroll
+            for (int e = 0; e < NS; e++) ai[k][bi][e] = 0.0;
+          }
+#pragma unroll
+          for (int e = 0; e < NS; e++) {
+            acca[k][bi][e] = 0.0;
+            accb[k][bi][e] = 0.0;
+          }
+        }
+      }
+      // Every group of the wave walks the same number of blocks when FUSED_D
+      // (the butterflies need all lanes); rows of different length are padded
+      // with inactive iterations.
+      int nblk = p1 - p0;
+      if constexpr (FUSED_D) {
+#pragma unroll
+        for (int m = L; m < WAVE; m <<= 1) nblk = max(nblk, __shfl_xor(nblk, m, WAVE));
+      }
+      for (int t = 0; t < nblk; t++) {
+        const int p = p0 + t;
+        const bool live = p < p1;
+        const int j = live ? ci[p] : 0;
+        const S* hb = hv + (size_t)(live ? p : 0) * GE::BBS;
+        double xb[KQ][B][NS];
+#pragma unroll
+        for (int bj = 0; bj < B; bj++) {
+          double hc[B * HS];  // block column bj
+#pragma unroll
+          for (int u = 0; u < B * HS; u++) hc[u] = live ? (double)hb[bj * B * HS + u] : 0.0;
+          const size_t ro = (size_t)(B * j + bj) * rs;
+#pragma unroll
+          for (int k = 0; k < KQ; k++) {
+            const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
+            double xa[NS];
+            if (live) {
+              ldd<S, NS>(am + off, xa);
+              if (vb1) {
+#pragma unroll
+                for (int e = 0; e < NS; e++) xb[k][bj][e] = xa[e];
+              } else {
+                ldd<S, NS>(bm1 + off, xb[k][bj]);
+              }
+            } else {
+#pragma unroll
+              for (int e = 0; e < NS; e++) {
+                xa[e] = 0.0;
+                xb[k][bj][e] = 0.0;
+              }
+            }
+#pragma unroll
+            for (int bi = 0; bi < B; bi++) {
+              const double hr = hc[bi * HS], hi = CX ? hc[bi * HS + HS - 1] : 0.0;
+              cmac<CX, NS>(acca[k][bi], hr, hi, xa);
+              cmac<CX, NS>(accb[k][bi], hr, hi, xb[k][bj]);
+            }
+          }
+        }
+        if constexpr (FUSED_D) {
+          // D_p(bi, bj) += dscale * hb1 * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s])
+          double own[GE::DOWN][HS];
+#pragma unroll
+          for (int o = 0; o < GE::DOWN; o++)
+#pragma unroll
+            for (int c = 0; c < HS; c++) own[o][c] = 0.0;
+#pragma unroll
+          for (int bj = 0; bj < B; bj++)
+#pragma unroll
+            for (int bi = 0; bi < B; bi++) {
+              double re = 0.0, im = 0.0;
+#pragma unroll
+              for (int k = 0; k < KQ; k++)
+#pragma unroll
+                for (int e = 0; e < NS; e += HS) {
+                  const double ar = ai[k][bi][e], br = xb[k][bj][e];
+                  re = fma(ar, br, re);
+                  if constexpr (CX) {
+                    const double aim = ai[k][bi][e + 1], bim = xb[k][bj][e + 1];
+                    re = fma(aim, bim, re);
+                    im = fma(aim, br, im);
+                    im = fma(-ar, bim, im);
+                  }
+                }
+              re = group_sum<L>(re);
+              if constexpr (CX) im = group_sum<L>(im);
+              const int de = bj * B + bi;  // column-major element index in the block
+              if (l == de % L) {
+                own[de / L][0] = re;
+                if constexpr (CX) own[de / L][HS - 1] = im;
+              }
+            }
+          if (live) {
+            const double f = dscale * hb1;
+#pragma unroll
+            for (int o = 0; o < GE::DOWN; o++) {
+              const int de = o * L + l;
+              if (de < GE::DE) {
+                double* d = dacc + ((size_t)p * GE::DE + de) * HS;
+#pragma unroll
+                for (int c = 0; c < HS; c++) d[c] = fma(f, own[o][c], d[c]);
+              }
+            }
+          }
+        }
+      }
+      if (!active) continue;
+#pragma unroll
+      for (int bi = 0; bi < B; bi++) {
+        const size_t ro = (size_t)(B * i + bi) * rs;
+#pragma unroll
+        for (int k = 0; k < KQ; k++) {
+          const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
+          double a1[NS], an[NS], bo[NS], b2v[NS];
+          ldd<S, NS>(am1 + off, a1);
+#pragma unroll
+          for (int e = 0; e < NS; e++) an[e] = rnd<S>(2.0 * acca[k][bi][e] - a1[e]);
+          std_<S, NS>(a_out + off, an);
+          if (bm2 != nullptr) {
+            ldd<S, NS>(bm2 + off, b2v);
+          } else {
+#pragma unroll
+            for (int e = 0; e < NS; e++) b2v[e] = sb2 * a1[e];
+          }
+#pragma unroll
+          for (int e = 0; e < NS; e++) {
+            double v = fma(2.0 * hb1, accb[k][bi][e], -b2v[e]);
+            v = fma(c4, ai[k][bi][e], v);
+            v = fma(c2, a1[e], v);
+            v = fma(w, an[e], v);
+            bo[e] = rnd<S>(v);
+          }
+          std_<S, NS>(b_out + off, bo);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sparse outer product on D's block pattern:
+//   D_p(bi, bj) += f * sum_s A[B i + bi, s] conj(Bm[B j + bj, s])
+// The operation of Engine_CPU::outer_product (engine_cpu.cpp:213-226) and of
+// the reference CUDA kernel (outer_product.cu:38-60); here one lane group per
+// D block row, KQ packets per lane, lane butterflies for the S reduction.
+// TA / TB are the scalar types of A and Bm (storage type or double for xi).
+// ---------------------------------------------------------------------------
+template <typename S, typename TA, typename TB, bool CX, int B, int L, int KQ>
+__global__ __launch_bounds__(WG) void k_outer(
+    const int* __restrict__ drp, const int* __restrict__ dci, const TA* __restrict__ A,
+    const TB* __restrict__ Bm, double* __restrict__ dacc, int nb, int qp, int chunk, double f) {
+  using GE = Geo<S, CX, B, L, KQ>;
+  constexpr int NS = GE::NS;  // scalars of one storage packet (defines the column chunk)
+  constexpr int HS = GE::HS;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = lane % L;
+  const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
+  const RowRange rr = wg_rows(nb, chunk);
+  const size_t rs = (size_t)qp * NS;  // scalars per row (same count for every type)
+
+  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+    const int i = base + g;
+    const bool active = i < rr.r1;
+    const int p0 = active ? drp[i] : 0;
+    const int p1 = active ? drp[i + 1] : 0;
+    int nblk = p1 - p0;
+#pragma unroll
+    for (int m = L; m < WAVE; m <<= 1) nblk = max(nblk, __shfl_xor(nblk, m, WAVE));
+    for (int t = 0; t < nblk; t++) {
+      const int p = p0 + t;
+      const bool live = p < p1;
+      const int j = live ? dci[p] : 0;
+      double own[GE::DOWN][HS];
+#pragma unroll
+      for (int o = 0; o < GE::DOWN; o++)
+#pragma unroll
+        for (int c = 0; c < HS; c++) own[o][c] = 0.0;
+      // Rows bi of the block are taken BI at a time to bound the registers of
+      // the partial sums (B*B*HS doubles would not fit for B = 8).
+      constexpr int BI = B <= 4 ? B : 2;
+#pragma unroll
+      for (int bi0 = 0; bi0 < B; bi0 += BI) {
+        double part[BI][B][HS];
+#pragma unroll
+        for (int bi = 0; bi < BI; bi++)
+#pragma unroll
+          for (int bj = 0; bj < B; bj++)
+#pragma unroll
+            for (int c = 0; c < HS; c++) part[bi][bj][c] = 0.0;
+        for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+          if (!live) break;
+#pragma unroll
+          for (int k = 0; k < KQ; k++) {
+            const size_t q = (size_t)(q0 + k * L + l) * NS;
+            double av[BI][NS], bv[B][NS];
+#pragma unroll
+            for (int bi = 0; bi < BI; bi++)
+              ldd<TA, NS>(A + (size_t)(B * i + bi0 + bi) * rs + q, av[bi]);
+#pragma unroll
+            for (int bj = 0; bj < B; bj++) ldd<TB, NS>(Bm + (size_t)(B * j + bj) * rs + q, bv[bj]);
+#pragma unroll
+            for (int bi = 0; bi < BI; bi++)
+#pragma unroll
+              for (int bj = 0; bj < B; bj++)
+#pragma unroll
+                for (int e = 0; e < NS; e += HS) {
+                  part[bi][bj][0] = fma(av[bi][e], bv[bj][e], part[bi][bj][0]);
+                  if constexpr (CX) {
+                    part[bi][bj][0] = fma(av[bi][e + 1], bv[bj][e + 1], part[bi][bj][0]);
+                    part[bi][bj][1] = fma(av[bi][e + 1], bv[bj][e], part[bi][bj][1]);
+                    part[bi][bj][1] = fma(-av[bi][e], bv[bj][e + 1], part[bi][bj][1]);
+                  }
+                }
+          }
+        }
+#pragma unroll
+        for (int bj = 0; bj < B; bj++)
+#pragma unroll
+          for (int bi = 0; bi < BI; bi++) {
+            const int de = bj * B + bi0 + bi;
+#pragma unroll
+            for (int c = 0; c < HS; c++) {
+              const double s = group_sum<L>(part[bi][bj][c]);
+              if (l == de % L) own[de / L][c] = s;
+            }
+          }
+      }
+      if (live) {
+#pragma unroll
+        for (int o = 0; o < GE::DOWN; o++) {
+          const int de = o * L + l;
+          if (de < GE::DE) {
+            double* d = dacc + ((size_t)p * GE::DE + de) * HS;
+#pragma unroll
+            for (int c = 0; c < HS; c++) d[c] = fma(f, own[o][c], d[c]);
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small helper kernels.
+// ---------------------------------------------------------------------------
+
+#ifdef FKD_DEFINE_HELPERS
+// Deterministic reduction of per-workgroup moment partials: out[2 s + c] =
+// sum_w partial[s][w][c], one workgroup per step s, fixed order.
+__global__ __launch_bounds__(WG) void k_reduce_partials(const double* __restrict__ partial,
+                                                        int nwg, double* __restrict__ out) {
+  const double* p = partial + (size_t)blockIdx.x * nwg * 2;
+  double a = 0.0, b = 0.0;
+  for (int w = threadIdx.x; w < nwg; w += WG) {
+    a += p[2 * w];
+    b += p[2 * w + 1];
+  }
+  wg_sum2(a, b, out + 2 * (size_t)blockIdx.x);
+}
+
+// Fill n doubles with zero (graph-capturable, unlike hipMemsetAsync).
+__global__ void k_zero(double* __restrict__ p, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    p[k] = 0.0;
+}
+
+#endif  // FKD_DEFINE_HELPERS
+
+// R (host type TH, column-major n x s) -> storage S, row-major n x sp, zero
+// padded.  Tiled through LDS: 64 rows x 16 columns per workgroup.
+template <typename TH, typename S, bool CX>
+__global__ __launch_bounds__(WG) void k_pack_R(const TH* __restrict__ src, S* __restrict__ dst,
+                                               int n, int s, int sp) {
+  constexpr int HS = CX ? 2 : 1;
+  __shared__ double tile[16][64 * HS + 1];
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 16;
+  // load: thread t reads row r0 + t % 64 of columns c0 + t / 64 + 4 k
+  for (int k = threadIdx.x; k < 64 * 16; k += WG) {
+    const int rr = k % 64, cc = k / 64;
+    const int r = r0 + rr, c = c0 + cc;
+#pragma unroll
+    for (int h = 0; h < HS; h++)
+      tile[cc][rr * HS + h] = (r < n && c < s) ? (double)src[((size_t)c * n + r) * HS + h] : 0.0;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 64 * 16; k += WG) {
+    const int cc = k % 16, rr = k / 16;
+    const int r = r0 + rr, c = c0 + cc;
+    if (r < n && c < sp) {
+#pragma unroll
+      for (int h = 0; h < HS; h++) dst[((size_t)r * sp + c) * HS + h] = (S)tile[cc][rr * HS + h];
+    }
+  }
+}
+
+// Hs values: host type TH -> S with (H - avg I)/mag applied, as
+// Engine_cuSPARSE::set_H does (engine_cusparse.cpp:711-727): divide every value
+// by mag, then subtract avg/mag on the diagonal of diagonal blocks.  Computed in
+// FP64 and rounded once to the storage type.
+template <typename TH, typename S, bool CX>
+__global__ void k_scale_H(const TH* __restrict__ src, S* __restrict__ dst,
+                          const int* __restrict__ row_of_block, const int* __restrict__ ci,
+                          int nnzb, int b, double mag, double shift) {
+  constexpr int HS = CX ? 2 : 1;
+  const size_t total = (size_t)nnzb * b * b;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = e / ((size_t)b * b);
+    const int r = (int)(e % ((size_t)b * b));
+    const int bj = r / b, bi = r % b;
+#pragma unroll
+    for (int h = 0; h < HS; h++) {
+      double v = (double)src[e * HS + h] / mag;
+      if (h == 0 && bi == bj && row_of_block[p] == ci[p]) v -= shift;
+      dst[e * HS + h] = (S)v;
+    }
+  }
+}
+
+// D_out_k(bi, bj) = ((D_k(bi, bj) + conj(D_t(k)(bj, bi))) / 2) * inv_mag, the
+// combined SpMatBsr::symmetrize + scale(1/mag) (spmat.cpp:171-197) written out
+// of place; t(k) is the index of block (j, i).
+template <bool CX>
+__global__ void k_symmetrize_scale(const double* __restrict__ din, double* __restrict__ dout,
+                                   const int* __restrict__ tpos, int nnzb, int b, double inv_mag) {
+  constexpr int HS = CX ? 2 : 1;
+  const size_t bb = (size_t)b * b;
+  const size_t total = (size_t)nnzb * bb;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t k = e / bb;
+    const int r = (int)(e % bb);
+    const int bj = r / b, bi = r % b;
+    const size_t e2 = (size_t)tpos[k] * bb + (size_t)bi * b + bj;
+    const double re = (din[e * HS] + din[e2 * HS]) / 2.0;
+    dout[e * HS] = re * inv_mag;
+    if constexpr (CX) {
+      const double im = (din[e * HS + 1] - din[e2 * HS + 1]) / 2.0;
+      dout[e * HS + 1] = im * inv_mag;
+    }
+  }
+}
+
+#ifdef FKD_DEFINE_HELPERS
+// ---------------------------------------------------------------------------
+// Lanczos helpers (FP64, one column): w -= a v1 + b v0 with |w|^2 partials;
+// v0 <- v1, v1 <- w / beta.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_lanczos_orth(double* __restrict__ w,
+                                                     const double* __restrict__ v1,
+                                                     const double* __restrict__ v0, size_t n,
+                                                     double a, double b,
+                                                     double* __restrict__ partial) {
+  double s = 0.0;
+  for (size_t k = (size_t)blockIdx.x * WG + threadIdx.x; k < n; k += (size_t)gridDim.x * WG) {
+    const double v = w[k] - a * v1[k] - b * v0[k];
+    w[k] = v;
+    s = fma(v, v, s);
+  }
+  wg_sum2(s, 0.0, partial + 2 * (size_t)blockIdx.x);
+}
+
+__global__ void k_lanczos_next(double* __restrict__ v0, double* __restrict__ v1,
+                               const double* __restrict__ w, size_t n, double inv_beta) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    v0[k] = v1[k];
+    v1[k] = w[k] * inv_beta;
+  }
+}
+
+#endif  // FKD_DEFINE_HELPERS
+
+}  // namespace fkd

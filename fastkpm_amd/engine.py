@@ -1,0 +1,239 @@
+"""Python mirror of the fkpm::Engine<T> interface (src/fastkpm.h:185-234) over
+the C-ABI of libfastkpm_hip.so.
+
+Same method names, argument meaning and call-order rules as the reference
+engine: set_R_* / set_R (transfer_R), set_H(H, es), moments(M),
+stoch_matrix(c, D), autodiff_matrix(c, D) (which requires moments(M) with the
+same M just before).  Errors raise FkpmError instead of aborting.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+from ._capi import call, lib
+
+_DTYPES = {
+    "float": (_capi.F32, np.float32, False),
+    "double": (_capi.F64, np.float64, False),
+    "cx_float": (_capi.C32, np.complex64, True),
+    "cx_double": (_capi.C64, np.complex128, True),
+}
+_PRECS = {"fp64": _capi.PREC_FP64, "fp32": _capi.PREC_FP32}
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class EnergyScale:
+    """fkpm::EnergyScale (src/fastkpm.h:112-122)."""
+    lo: float
+    hi: float
+
+    def avg(self) -> float:
+        return (self.hi + self.lo) / 2.0
+
+    def mag(self) -> float:
+        return (self.hi - self.lo) / 2.0
+
+    def scale(self, x):
+        return (x - self.avg()) / self.mag()
+
+    def unscale(self, x):
+        return x * self.mag() + self.avg()
+
+
+@dataclass
+class SpMatBsr:
+    """Block sparse row matrix, the fields of fkpm::SpMatBsr (src/fastkpm.h:66-74).
+
+    val holds nnzb*b*b values, each b x b block column-major (element (bi, bj)
+    of block k at k*b*b + bj*b + bi), block rows sorted by column.
+    """
+    n_rows: int
+    n_cols: int
+    b_len: int
+    row_idx: np.ndarray
+    col_idx: np.ndarray
+    row_ptr: np.ndarray
+    val: np.ndarray = field(default=None)
+
+    def n_blocks(self) -> int:
+        return int(len(self.row_idx))
+
+    def zeros(self) -> None:
+        self.val[:] = 0
+
+    def like(self, dtype=None) -> "SpMatBsr":
+        """Same pattern, zero values (the caller-owned D of stoch/autodiff)."""
+        dt = self.val.dtype if dtype is None else dtype
+        return SpMatBsr(self.n_rows, self.n_cols, self.b_len, self.row_idx.copy(),
+                        self.col_idx.copy(), self.row_ptr.copy(),
+                        np.zeros(self.n_blocks() * self.b_len ** 2, dtype=dt))
+
+    def find_index(self, i: int, j: int) -> int:
+        lo, hi = int(self.row_ptr[i]), int(self.row_ptr[i + 1])
+        hits = np.nonzero(self.col_idx[lo:hi] == j)[0]
+        if len(hits) == 0:
+            raise KeyError(f"could not find block index ({i},{j}) in BSR matrix")
+        return lo + int(hits[0])
+
+    def block(self, k: int) -> np.ndarray:
+        b = self.b_len
+        return self.val[k * b * b:(k + 1) * b * b].reshape(b, b).T
+
+    def to_dense(self) -> np.ndarray:
+        b = self.b_len
+        out = np.zeros((self.n_rows * b, self.n_cols * b), dtype=self.val.dtype)
+        for k in range(self.n_blocks()):
+            i, j = int(self.row_idx[k]), int(self.col_idx[k])
+            out[b * i:b * i + b, b * j:b * j + b] += self.block(k)
+        return out
+
+
+class HipEngine:
+    """fkpm::Engine<T> on one HIP device.
+
+    dtype: 'float', 'double', 'cx_float', 'cx_double' (the engine's T).
+    prec:  'fp64' (vectors and H in FP64) or 'fp32' (FP32 recursion storage
+           with FP64 row sums, moment dots, xi and D accumulation).
+    """
+
+    def __init__(self, dtype: str = "cx_double", prec: str = "fp64", device: int = 0):
+        if dtype not in _DTYPES:
+            raise ValueError(f"dtype must be one of {list(_DTYPES)}")
+        self.dtype = dtype
+        self._code, self.np_dtype, self.is_complex = _DTYPES[dtype]
+        self.prec = prec
+        h = C.c_void_p()
+        call("fkpm_dev_open", device, self._code, _PRECS[prec], C.byref(h))
+        self._h = h
+        self.R = None
+        self.es = None
+        self._d_pattern_key = None
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().fkpm_dev_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stream(self) -> int:
+        return int(lib().fkpm_stream(self._h) or 0)
+
+    # -- R ------------------------------------------------------------------
+    def set_R(self, R: np.ndarray) -> None:
+        """Set Engine::R (n x s) and transfer it to the device (transfer_R)."""
+        R = np.asfortranarray(R, dtype=self.np_dtype)
+        n, s = R.shape
+        self.R = R
+        call("fkpm_set_R", self._h, n, s, R.ctypes.data_as(C.c_void_p))
+
+    # -- spectrum -----------------------------------------------------------
+    def energy_scale(self, H: SpMatBsr, extend: float, iters: int = 128,
+                     v_start: np.ndarray | None = None) -> EnergyScale:
+        """Lanczos bounds of H (Engine::energy_scale, src/fastkpm.h:193-194)."""
+        n = H.n_rows * H.b_len
+        if v_start is None:
+            rs = np.random.default_rng(0)
+            v_start = rs.standard_normal(n)
+            if self.is_complex:
+                v_start = v_start + 1j * rs.standard_normal(n)
+        v = np.ascontiguousarray(v_start, dtype=self.np_dtype)
+        val = np.ascontiguousarray(H.val, dtype=self.np_dtype)
+        rp = np.ascontiguousarray(H.row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(H.col_idx, dtype=np.int32)
+        lo, hi = C.c_double(), C.c_double()
+        call("fkpm_energy_scale", self._h, H.n_rows, H.b_len, H.n_blocks(), _iptr(rp), _iptr(ci),
+             val.ctypes.data_as(C.c_void_p), float(extend), int(iters),
+             v.ctypes.data_as(C.c_void_p), C.byref(lo), C.byref(hi))
+        return EnergyScale(lo.value, hi.value)
+
+    def set_H(self, H: SpMatBsr, es: EnergyScale) -> None:
+        if H.n_rows != H.n_cols:
+            raise ValueError("H must be square")
+        self.es = es
+        val = np.ascontiguousarray(H.val, dtype=self.np_dtype)
+        rp = np.ascontiguousarray(H.row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(H.col_idx, dtype=np.int32)
+        call("fkpm_set_H", self._h, H.n_rows, H.b_len, H.n_blocks(), _iptr(rp), _iptr(ci),
+             val.ctypes.data_as(C.c_void_p), float(es.avg()), float(es.mag()))
+
+    # -- recursions ---------------------------------------------------------
+    def moments(self, M: int) -> np.ndarray:
+        mu = np.zeros(M, dtype=np.float64)
+        call("fkpm_moments", self._h, int(M), _dptr(mu))
+        return mu
+
+    def _set_D(self, D: SpMatBsr) -> None:
+        rp = np.ascontiguousarray(D.row_ptr, dtype=np.int32)
+        ri = np.ascontiguousarray(D.row_idx, dtype=np.int32)
+        ci = np.ascontiguousarray(D.col_idx, dtype=np.int32)
+        key = (D.n_rows, rp.tobytes(), ci.tobytes())
+        if key != self._d_pattern_key:
+            call("fkpm_set_D_pattern", self._h, D.n_rows, D.n_blocks(), _iptr(rp), _iptr(ri),
+                 _iptr(ci))
+            self._d_pattern_key = key
+
+    def _D_out(self, D: SpMatBsr, keep_on_device: bool):
+        if keep_on_device:
+            return None
+        if D.val is None or D.val.dtype != self.np_dtype or not D.val.flags.c_contiguous \
+                or D.val.size != D.n_blocks() * D.b_len ** 2:
+            D.val = np.zeros(D.n_blocks() * D.b_len ** 2, dtype=self.np_dtype)
+        return D.val.ctypes.data_as(C.c_void_p)
+
+    def stoch_matrix(self, c, D: SpMatBsr, keep_on_device: bool = False) -> None:
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self._set_D(D)
+        call("fkpm_stoch_matrix", self._h, len(c), _dptr(c), self._D_out(D, keep_on_device))
+
+    def autodiff_matrix(self, c, D: SpMatBsr, keep_on_device: bool = False) -> None:
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self._set_D(D)
+        call("fkpm_autodiff_matrix", self._h, len(c), _dptr(c), self._D_out(D, keep_on_device))
+
+    def get_D(self, D: SpMatBsr) -> None:
+        call("fkpm_get_D", self._h, self._D_out(D, False))
+
+    # -- introspection ------------------------------------------------------
+    def set_timing(self, on: bool) -> None:
+        call("fkpm_set_timing", self._h, 1 if on else 0)
+
+    def timing(self) -> dict:
+        v = [C.c_double(), C.c_int(), C.c_double(), C.c_int(), C.c_double(), C.c_int()]
+        call("fkpm_get_timing", self._h, *[C.byref(x) for x in v])
+        return {"moments_ms": v[0].value, "moments_launches": v[1].value,
+                "stoch_ms": v[2].value, "stoch_launches": v[3].value,
+                "autodiff_ms": v[4].value, "autodiff_launches": v[5].value}
+
+    def config(self) -> dict:
+        v = [C.c_int() for _ in range(5)]
+        call("fkpm_get_config", self._h, *[C.byref(x) for x in v])
+        return {"lanes": v[0].value, "kq": v[1].value, "s_padded": v[2].value,
+                "n_wg": v[3].value, "fused_autodiff": bool(v[4].value)}
+
+    def set_geometry(self, lanes: int = 0, kq: int = 0, max_wg: int = 0) -> None:
+        call("fkpm_set_geometry", self._h, int(lanes), int(kq), int(max_wg))
+
+
+def device_count() -> int:
+    return int(lib().fkpm_device_count())

@@ -1,0 +1,139 @@
+/*
+ * fastkpm_hip.h -- C-ABI of the MI355X (gfx950) stochastic Chebyshev engine.
+ *
+ * This is the drop-in boundary below the C++ fkpm::Engine<T> interface
+ * (include/fastkpm.h).  It replaces the device half of the reference GPU
+ * engine, Engine_cuSPARSE (src/engine_cusparse.cpp:486-1040), and its one CUDA
+ * kernel (src/outer_product.cu:38-68).  Plain C types only: no C++ or torch
+ * types cross it.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative FKPM_E* code; the
+ *    message of the last failure on the calling thread is fkpm_last_error().
+ *    (The reference aborts / exits instead, engine_cusparse.cpp:25,60-66.)
+ *  - A handle owns one HIP device, one HIP stream and all device buffers.  A
+ *    handle is not thread-safe (neither is a reference Engine).
+ *  - Host arrays are caller-owned.  Value arrays hold the engine element type
+ *    chosen at fkpm_dev_open (FKPM_F32, FKPM_F64, FKPM_C32, FKPM_C64 = float,
+ *    double, std::complex<float>, std::complex<double>), complex values
+ *    interleaved (re, im).
+ *  - BSR blocks are b x b, column-major inside a block, block rows sorted by
+ *    column, as SpMatBsr (src/fastkpm.h:64-106).
+ *  - Calls are synchronous with respect to the host arrays they take: results
+ *    are in host memory on return, unless a NULL result pointer asks for the
+ *    result to stay device-resident (fkpm_get_D copies it later).
+ */
+#ifndef FASTKPM_HIP_H
+#define FASTKPM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types of host arrays (the T of Engine<T>, src/engine.cpp:118-121) */
+#define FKPM_F32 0
+#define FKPM_F64 1
+#define FKPM_C32 2
+#define FKPM_C64 3
+
+/* device storage precision of the Chebyshev vectors.  Accumulation of moment
+ * dots, xi and D is FP64 in both. */
+#define FKPM_PREC_FP64 0 /* vectors and H in FP64 */
+#define FKPM_PREC_FP32 1 /* vectors and H in FP32, row sums and dots in FP64 */
+
+/* error codes */
+#define FKPM_OK 0
+#define FKPM_E_ARG (-1)     /* invalid argument / shape mismatch */
+#define FKPM_E_STATE (-2)   /* call order violated (e.g. autodiff without moments) */
+#define FKPM_E_HIP (-3)     /* HIP runtime error */
+#define FKPM_E_NODEV (-4)   /* no usable HIP device */
+#define FKPM_E_UNSUP (-5)   /* unsupported configuration */
+
+typedef struct fkpm_handle fkpm_handle;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fkpm_last_error(void);
+
+/* Number of visible HIP devices (replaces the cudaGetDeviceCount probe of
+ * mk_engine_cuSPARSE, engine_cusparse.cpp:1171-1215).  Returns 0 when none. */
+int fkpm_device_count(void);
+
+/* Open an engine on `device` for element type `dtype` and storage precision
+ * `prec`.  Replaces the Engine_cuSPARSE constructor (engine_cusparse.cpp:514-523).
+ * FKPM_PREC_FP32 with an FP64 dtype runs the FP32-mixed recursion on FP64
+ * host data. */
+int fkpm_dev_open(int device, int dtype, int prec, fkpm_handle** out);
+
+/* Release all device memory and the stream (engine_cusparse.cpp:525-531). */
+int fkpm_dev_close(fkpm_handle* h);
+
+/* The HIP stream (hipStream_t) all work of this handle is queued on. */
+void* fkpm_stream(fkpm_handle* h);
+
+/* Lanczos estimate of the spectral bounds of the unscaled H, FP64 on the
+ * device; the iters x iters tridiagonal eigenproblem is solved on the host.
+ * Replaces Engine_cuSPARSE::energy_scale (engine_cusparse.cpp:598-673).
+ * v_start: n = nb*b start-vector entries of the engine type (the reference
+ * draws it with arma::randn).  Writes lo/hi = [lmin - slack, lmax + slack],
+ * slack = extend*(lmax - lmin). */
+int fkpm_energy_scale(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr,
+                      const int* col_idx, const void* val, double extend, int iters,
+                      const void* v_start, double* lo, double* hi);
+
+/* Upload the scaled Hamiltonian Hs = (H - avg I)/mag.  Every diagonal block
+ * must be present.  Replaces Engine_cuSPARSE::set_H (engine_cusparse.cpp:701-730). */
+int fkpm_set_H(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr,
+               const int* col_idx, const void* val, double avg, double mag);
+
+/* Upload R (n x s, column-major, the layout of Engine::R) and allocate the
+ * recursion state.  Replaces Engine_cuSPARSE::transfer_R (engine_cusparse.cpp:675-699). */
+int fkpm_set_R(fkpm_handle* h, int n, int s, const void* R_colmajor);
+
+/* Chebyshev moments mu[0..M-1], M even, with the two-moments-per-product
+ * identity.  Leaves alpha_{M/2-1}, alpha_{M/2} on the device for
+ * fkpm_autodiff_matrix.  Replaces Engine_cuSPARSE::moments (engine_cusparse.cpp:755-791). */
+int fkpm_moments(fkpm_handle* h, int M, double* mu);
+
+/* Block pattern of D (nbD block rows; row_ptr[nbD+1], row_idx/col_idx[nnzbD]),
+ * the sparsity structure of SpMatBsr D in stoch_matrix/autodiff_matrix.  Kept
+ * until replaced; a pattern equal to H's enables the fused autodiff kernel. */
+int fkpm_set_D_pattern(fkpm_handle* h, int nbD, int nnzbD, const int* row_ptr,
+                       const int* row_idx, const int* col_idx);
+
+/* D = (R xi^+ + xi R^+)/2 on D's pattern with xi = sum_{m<M} c_m T_m(Hs) R.
+ * D_val: nnzbD*b*b values of the engine type, or NULL to leave D on the
+ * device.  Invalidates the moments state.  Replaces
+ * Engine_cuSPARSE::stoch_matrix (engine_cusparse.cpp:910-953). */
+int fkpm_stoch_matrix(fkpm_handle* h, int M, const double* c, void* D_val);
+
+/* D = d tr g / dH^T on D's pattern by reverse-mode differentiation of the
+ * moment recursion, symmetrized and scaled by 1/mag.  Requires fkpm_moments
+ * with the same M just before.  D_val as in fkpm_stoch_matrix.  Replaces
+ * Engine_cuSPARSE::autodiff_matrix (engine_cusparse.cpp:955-1039). */
+int fkpm_autodiff_matrix(fkpm_handle* h, int M, const double* c, void* D_val);
+
+/* Copy the last device-resident D to the host (engine type). */
+int fkpm_get_D(fkpm_handle* h, void* D_val);
+
+/* Per-call timing of the dominant recursion kernels, measured with HIP events
+ * on the handle's stream when enabled: total milliseconds and launch counts of
+ * the Chebyshev step kernels of the last fkpm_moments / fkpm_stoch_matrix /
+ * fkpm_autodiff_matrix call. */
+int fkpm_set_timing(fkpm_handle* h, int on);
+int fkpm_get_timing(fkpm_handle* h, double* ms_moments, int* n_moments, double* ms_stoch,
+                    int* n_stoch, double* ms_autodiff, int* n_autodiff);
+
+/* Kernel geometry in use: lanes per block row, packets per lane, padded S,
+ * workgroups per step launch, and whether the fused autodiff path is active. */
+int fkpm_get_config(fkpm_handle* h, int* lanes, int* kq, int* s_padded, int* n_wg,
+                    int* fused_autodiff);
+
+/* Force a kernel geometry (lanes per block row, packets per lane; 0 = pick
+ * automatically) for the next fkpm_set_R.  Used by the autotuner. */
+int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTKPM_HIP_H */

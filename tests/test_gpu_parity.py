@@ -1,0 +1,192 @@
+"""Parity of the HIP engine (through the C-ABI) with the CPU oracle on
+identical seeded inputs.  FP64: 1e-10 relative (norm-wise, max|a-b|/max|b|).
+FP32-mixed: the tolerance stated in each test."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import fermi_coeffs, rel_err, seeded_R
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def _engine(dtype="cx_double", prec="fp64"):
+    from fastkpm_amd import HipEngine
+    return HipEngine(dtype, prec)
+
+
+def _case(oracle, shape, b, seed, J=0.0, W=0.0, orbital_blocks=False, periods=None,
+          real=False):
+    from fastkpm_amd import lattice
+    dtype = np.float64 if real else np.complex128
+    H = lattice.lattice(shape, b=b, J=J, W=W, orbital_blocks=orbital_blocks, seed=seed,
+                        dtype=dtype)
+    periods = periods or tuple(2 for _ in shape)
+    groups = lattice.colouring(shape, periods, b)
+    R = seeded_R(oracle, groups, seed + 11, is_complex=not real)
+    if real:
+        R = R.real
+    return H, R
+
+
+def _run_pair(oracle, H, R, lo, hi, M, dtype="cx_double", prec="fp64", D=None):
+    eng = _engine(dtype, prec)
+    es_lo, es_hi = lo, hi
+    from fastkpm_amd import EnergyScale
+    eng.set_R(R)
+    eng.set_H(H, EnergyScale(es_lo, es_hi))
+    orc = oracle.Engine()
+    orc.set_H(H, es_lo, es_hi)
+    orc.set_R(R)
+    D = D if D is not None else H.like()
+    return eng, orc, D
+
+
+CASES = [
+    # name, shape, b, J, W, orbital_blocks, periods, real
+    ("square_b1_real", (8, 8), 1, 0.0, 1.0, False, (4, 4), True),
+    ("square_b1_cx", (8, 8), 1, 0.0, 2.0, False, (4, 4), False),
+    ("kondo_b2", (8, 8), 2, 2.0, 0.0, False, (4, 4), False),
+    ("anderson_kondo_3d_b2", (4, 4, 4), 2, 1.0, 4.0, False, (2, 2, 2), False),
+    ("orbital_b4", (4, 4), 4, 0.0, 0.0, True, (2, 2), False),
+    ("orbital_b8", (4, 4, 2), 8, 0.0, 0.0, True, (2, 2, 2), False),
+]
+
+
+@pytest.mark.parametrize("name,shape,b,J,W,ob,periods,real", CASES, ids=[c[0] for c in CASES])
+def test_fp64_moments_stoch_autodiff(oracle, name, shape, b, J, W, ob, periods, real):
+    H, R = _case(oracle, shape, b, seed=7, J=J, W=W, orbital_blocks=ob, periods=periods, real=real)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    lo, hi = lam[0] - 0.1, lam[-1] + 0.1
+    M = 64
+    dtype = "double" if real else "cx_double"
+    eng, orc, D = _run_pair(oracle, H, R, lo, hi, M, dtype=dtype)
+    c = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
+
+    mu_g = eng.moments(M)
+    mu_o = orc.moments(M)
+    assert rel_err(mu_g, mu_o) < FP64_TOL, (name, rel_err(mu_g, mu_o))
+
+    eng.autodiff_matrix(c, D)
+    d_o = orc.autodiff_matrix(c, D)
+    assert rel_err(D.val, d_o) < FP64_TOL, (name, "autodiff", rel_err(D.val, d_o))
+
+    D2 = H.like()
+    eng.stoch_matrix(c, D2)
+    s_o = orc.stoch_matrix(c, D2)
+    assert rel_err(D2.val, s_o) < FP64_TOL, (name, "stoch", rel_err(D2.val, s_o))
+
+
+@pytest.mark.parametrize("S_cols", [1, 3, 5, 16, 33])
+def test_fp64_ragged_columns(oracle, S_cols):
+    """R with a column count that is not a multiple of any lane geometry."""
+    from fastkpm_amd import lattice, EnergyScale
+    H = lattice.lattice((6, 6), b=2, J=1.5, seed=3)
+    n = H.n_rows * H.b_len
+    rng = oracle.MT19937(99)
+    R = oracle.set_R_uncorrelated(n, S_cols, rng)
+    lo, hi = -6.0, 6.0
+    eng = _engine()
+    eng.set_R(R)
+    eng.set_H(H, EnergyScale(lo, hi))
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    M = 40
+    assert rel_err(eng.moments(M), orc.moments(M)) < FP64_TOL
+    c = fermi_coeffs(oracle, M, lo, hi)
+    D = H.like()
+    eng.autodiff_matrix(c, D)
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < FP64_TOL
+
+
+def test_fp64_foreign_D_pattern(oracle):
+    """D on a pattern other than H's (second neighbours): the unfused path."""
+    from fastkpm_amd import lattice, EnergyScale
+    shape = (6, 6)
+    H = lattice.lattice(shape, b=2, J=2.0, seed=5)
+    # pattern: on-site + next-nearest (diagonal) neighbours, symmetric
+    P = lattice.lattice(shape, b=2, t=0.0, seed=5)
+    nsite = 36
+    rows, cols = [], []
+    for s in range(nsite):
+        x, y = s % 6, s // 6
+        for dx, dy in ((0, 0), (1, 1), (-1, -1), (1, -1), (-1, 1)):
+            rows.append(s)
+            cols.append(((x + dx) % 6) + 6 * ((y + dy) % 6))
+    D = lattice.coo_to_bsr(nsite, 2, rows, cols, np.zeros((len(rows), 2, 2)))
+    del P
+    groups = lattice.colouring(shape, (3, 3), 2)
+    R = seeded_R(oracle, groups, 4)
+    lo, hi = -7.0, 7.0
+    eng = _engine()
+    eng.set_R(R)
+    eng.set_H(H, EnergyScale(lo, hi))
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    M = 50
+    eng.moments(M)
+    orc.moments(M)
+    c = fermi_coeffs(oracle, M, lo, hi)
+    eng.autodiff_matrix(c, D)
+    assert eng.config()["fused_autodiff"] is False
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < FP64_TOL
+    eng.stoch_matrix(c, D)
+    assert rel_err(D.val, orc.stoch_matrix(c, D)) < FP64_TOL
+
+
+def test_fp32_mixed_energy(oracle):
+    """FP32 recursion storage, FP64 accumulation: energy within 1e-6 relative."""
+    from fastkpm_amd import lattice, EnergyScale
+    H = lattice.lattice((16, 16), b=2, J=2.0, seed=1)
+    groups = lattice.colouring((16, 16), (4, 8), 2)
+    R = seeded_R(oracle, groups, 21)
+    lo, hi = -6.5, 6.5
+    M = 400
+    eng = _engine("cx_double", "fp32")
+    eng.set_R(R)
+    eng.set_H(H, EnergyScale(lo, hi))
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    mu_g = eng.moments(M)
+    mu_o = orc.moments(M)
+    c = fermi_coeffs(oracle, M, lo, hi)
+    e_g = float(np.dot(c, mu_g))
+    e_o = float(np.dot(c, mu_o))
+    assert abs(e_g - e_o) / abs(e_o) < 1e-6, (e_g, e_o)
+    D = H.like()
+    eng.autodiff_matrix(c, D)
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < 1e-4
+
+
+def test_energy_scale_matches_oracle(oracle):
+    from fastkpm_amd import lattice
+    H = lattice.lattice((8, 8), b=2, J=2.0, seed=2)
+    n = H.n_rows * H.b_len
+    rs = np.random.default_rng(5)
+    v = rs.standard_normal(n) + 1j * rs.standard_normal(n)
+    eng = _engine()
+    es = eng.energy_scale(H, 0.01, 64, v_start=v)
+    lo, hi, _, _ = oracle.energy_scale(H, 0.01, 64, v)
+    assert abs(es.lo - lo) < 1e-8 * abs(lo) and abs(es.hi - hi) < 1e-8 * abs(hi)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    assert es.lo <= lam[0] and es.hi >= lam[-1]
+
+
+def test_autodiff_requires_moments(oracle):
+    from fastkpm_amd import lattice, EnergyScale
+    from fastkpm_amd._capi import FkpmError
+    H = lattice.lattice((4, 4), b=1)
+    eng = _engine()
+    eng.set_R(np.eye(16))
+    eng.set_H(H, EnergyScale(-4.1, 4.1))
+    with pytest.raises(FkpmError):
+        eng.autodiff_matrix(np.ones(8), H.like())
+    eng.moments(8)
+    with pytest.raises(FkpmError):  # different M
+        eng.autodiff_matrix(np.ones(10), H.like())
