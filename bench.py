@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X stochastic Chebyshev engine.
+
+One step = one full KPM solve of the workload with inputs resident in HBM:
+  moments(M) -> E = sum c_m mu_m,
+  autodiff_matrix(c_E, D) (dE/dH on H's pattern, left on the device),
+  stoch_matrix(c_f, D')   (D_ij = f(H)_ij on H's pattern, left on the device).
+value = N*M*S / (seconds per step): Chebyshev vector updates per second.
+
+Default workload: Kondo lattice 256x256 sites x 2 spin orbitals (N = 131072,
+complex 2x2 BSR blocks, hopping -I2, on-site -J s_i.sigma with J = 2 and
+seeded random unit spins), M = 1000, S = 64 probing colours (4x8 spatial
+colouring x 2 orbitals), Fermi function at mu = 0, kT = 0.01, FP64.
+
+Prints one JSON line.  The CPU baseline times the FP64 oracle (single thread)
+on the same workload with M' moments (a bounded sample) and reports N*M'*S/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("N·M·S Chebyshev updates/s, FP32-mixed vs FP64 at ≤1e-6 energy err; "
+          "p99 lat")
+HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(name: str, seed: int):
+    from fastkpm_amd import lattice
+    H, groups, cfg = lattice.make_config(name, seed=seed)
+    return H, groups, cfg
+
+
+def algorithmic_bytes(H, N: int, S: int, v: int, vacc: int = 16):
+    """Algorithmic HBM bytes per launch of each step kernel (S logical columns,
+    v bytes per vector element, vacc bytes per FP64 accumulator element)."""
+    b = H.b_len
+    nnzb = H.n_blocks()
+    h_bytes = nnzb * b * b * v + 4 * nnzb + 4 * (H.n_rows + 1)
+    vec = N * S * v
+    acc = N * S * vacc
+    d_bytes = nnzb * b * b * vacc
+    return {
+        # read alpha_m, alpha_{m-1}, write alpha_{m+1}, read H
+        "moments": 3 * vec + h_bytes,
+        # + read-modify-write of xi (FP64)
+        "stoch": 3 * vec + 2 * acc + h_bytes,
+        # read alpha_m, alpha_{m+1}, beta_{m+1}, beta_{m+2}; write alpha_{m-1},
+        # beta_m; read H; read-modify-write D (FP64)
+        "autodiff": 6 * vec + h_bytes + 2 * d_bytes,
+    }
+
+
+def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
+    from fastkpm_amd import HipEngine, host
+    M = cfg["M"]
+    eng = HipEngine("cx_double", prec)
+    es = eng.energy_scale(H, 0.01)
+    eng.set_R_correlated(groups, host.RNG(seed))
+    eng.set_H(H, es)
+    S = eng.R.shape[1]
+    N = H.n_rows * H.b_len
+    Mq = 2 * M
+    cE = host.expansion_coefficients(M, Mq, host.FERMI_ENERGY, cfg["kT"], cfg["mu"], es.lo, es.hi)
+    cF = host.expansion_coefficients(M, Mq, host.FERMI_DENSITY, cfg["kT"], cfg["mu"], es.lo, es.hi)
+    Dg = H.like()
+    Ds = H.like()
+
+    def step():
+        mu = eng.moments(M)
+        eng.autodiff_matrix(cE, Dg, keep_on_device=True)
+        eng.stoch_matrix(cF, Ds, keep_on_device=True)
+        return mu
+
+    for _ in range(warmup):
+        step()
+    eng.set_timing(True)
+    tot = {"moments": [0.0, 0], "autodiff": [0.0, 0], "stoch": [0.0, 0]}
+    times = []
+    eng.synchronize()
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        mu = eng.moments(M)
+        tm = eng.timing()
+        eng.autodiff_matrix(cE, Dg, keep_on_device=True)
+        ta = eng.timing()
+        eng.stoch_matrix(cF, Ds, keep_on_device=True)
+        ts = eng.timing()
+        times.append(time.perf_counter() - t0)
+        tot["moments"][0] += tm["moments_ms"]
+        tot["moments"][1] += tm["moments_launches"]
+        tot["autodiff"][0] += ta["autodiff_ms"]
+        tot["autodiff"][1] += ta["autodiff_launches"]
+        tot["stoch"][0] += ts["stoch_ms"]
+        tot["stoch"][1] += ts["stoch_launches"]
+    eng.synchronize()
+    wall = time.perf_counter() - t_all
+    eng.set_timing(False)
+    energy = float(np.dot(cE, mu))
+    eng.get_D(Ds)  # the device holds the last computed D: stoch_matrix's
+    # copy of the gradient D: one more moments + autodiff_matrix to the host
+    eng.moments(M)
+    eng.autodiff_matrix(cE, Dg)
+    return dict(eng=eng, es=es, N=N, S=S, M=M, mu=mu, energy=energy, wall=wall, times=times,
+                kernel_ms=tot, Dg=Dg.val.copy(), Ds=Ds.val.copy(), config=eng.config())
+
+
+def cpu_baseline(H, groups, cfg, es, seed: int, m_sample: int):
+    """The FP64 oracle (the reference CPU algorithm restated in C, one thread)
+    on the same H, R and coefficients with M' = m_sample moments."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py
+    from fastkpm_amd import host
+    R = host.set_R_correlated("cx_double", groups, host.RNG(seed))
+    S = R.shape[1]
+    N = H.n_rows * H.b_len
+    cE = host.expansion_coefficients(m_sample, 2 * m_sample, host.FERMI_ENERGY, cfg["kT"],
+                                     cfg["mu"], es.lo, es.hi)
+    cF = host.expansion_coefficients(m_sample, 2 * m_sample, host.FERMI_DENSITY, cfg["kT"],
+                                     cfg["mu"], es.lo, es.hi)
+    orc = oracle_py.Engine()
+    orc.set_H(H, es.lo, es.hi)
+    orc.set_R(R)
+    D = H.like()
+    t0 = time.perf_counter()
+    orc.moments(m_sample)
+    orc.autodiff_matrix(cE, D)
+    orc.stoch_matrix(cF, D)
+    dt = time.perf_counter() - t0
+    return {"value": N * m_sample * S / dt, "unit": "updates/s", "cores": 1, "kind": "port",
+            "sample": (f"same workload (N={N}, S={S}) with M'={m_sample} moments: one "
+                       f"moments+autodiff_matrix+stoch_matrix pass of the FP64 oracle "
+                       f"(oracle/oracle.c, one thread) in {dt:.2f} s; host has "
+                       f"{os.cpu_count()} cores")}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-sample-moments", type=int, default=24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-mixed comparison")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the engine runs on a single device "
+              "(S-column sharding over several GPUs is designed, not built); rerun with --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    from fastkpm_amd import device_count
+    if device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+
+    _log(f"building workload {args.config}")
+    H, groups, cfg = build_workload(args.config, args.seed)
+    _log(f"N={H.n_rows * H.b_len} nnzb={H.n_blocks()} M={cfg['M']}; {args.precision} run")
+    r = run_gpu(H, groups, cfg, args.precision, args.steps, args.warmup, args.seed)
+    N, S, M = r["N"], r["S"], r["M"]
+    ms_per_step = 1e3 * r["wall"] / args.steps
+    value = N * M * S / (r["wall"] / args.steps)
+    v = 16 if args.precision == "fp64" else 8
+    abytes = algorithmic_bytes(H, N, S, v)
+    kern = {}
+    for k, (ms, n) in r["kernel_ms"].items():
+        if n == 0:
+            continue
+        avg_us = 1e3 * ms / n
+        gbs = abytes[k] / (avg_us * 1e-6) / 1e9
+        kern[k] = {"launches_per_step": n // args.steps, "avg_launch_us": round(avg_us, 3),
+                   "bytes_per_launch": int(abytes[k]), "achieved_GBs": round(gbs, 1),
+                   "frac": round(gbs / HBM_PEAK_GBS, 4),
+                   "ms_per_step": round(ms / args.steps, 3)}
+    dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    cfgk = r["config"]
+    names = {"moments": "k_step_moments", "stoch": "k_step_xi", "autodiff": "k_step_autodiff"}
+    roof = {"bound": "hbm", "achieved": kern[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None,
+            "kernel": f"{names[dom]} (b={H.b_len}, lanes={cfgk['lanes']}, kq={cfgk['kq']})",
+            "bytes_per_launch": kern[dom]["bytes_per_launch"],
+            "avg_launch_us": kern[dom]["avg_launch_us"]}
+    times_ms = np.array(r["times"]) * 1e3
+    out = {
+        "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32", "data": "synthetic",
+        "config": {"workload": (f"{args.config}: Kondo lattice 256x256x2 orbitals, complex 2x2 BSR, "
+                                f"J=2, M={M}, S={S}, kT=0.01; step = moments + autodiff_matrix "
+                                f"+ stoch_matrix"),
+                   "N": N, "M": M, "S": S, "b": H.b_len, "nnzb": H.n_blocks(),
+                   "precision": args.precision, "engine_T": "cx_double"},
+        "latency_ms": {"p50": round(float(np.percentile(times_ms, 50)), 3),
+                       "p99": round(float(np.percentile(times_ms, 99)), 3)},
+        "roofline": roof,
+        "kernels": kern,
+        "energy": r["energy"],
+    }
+    if not args.no_fp32 and args.precision == "fp64":
+        _log("fp32-mixed run")
+        r32 = run_gpu(H, groups, cfg, "fp32", args.steps, args.warmup, args.seed)
+        t32 = r32["wall"] / args.steps
+        e_err = abs(r32["energy"] - r["energy"]) / abs(r["energy"])
+        dg_err = float(np.max(np.abs(r32["Dg"] - r["Dg"])) / np.max(np.abs(r["Dg"])))
+        ds_err = float(np.max(np.abs(r32["Ds"] - r["Ds"])) / np.max(np.abs(r["Ds"])))
+        out["fp32_mixed"] = {"value": N * M * S / t32, "ms_per_step": round(1e3 * t32, 3),
+                             "speedup_vs_fp64": round(r["wall"] / r32["wall"], 3),
+                             "energy_rel_err_vs_fp64": e_err,
+                             "dEdH_rel_err_vs_fp64": dg_err, "Dij_rel_err_vs_fp64": ds_err}
+        del r32
+    if not args.no_cpu_baseline:
+        _log(f"cpu baseline (oracle, M'={args.cpu_sample_moments})")
+        out["cpu_baseline"] = cpu_baseline(H, groups, cfg, r["es"], args.seed,
+                                           args.cpu_sample_moments)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -26,6 +26,7 @@ SIGNATURES = {
     "fkpm_dev_open": (_I, [_I, _I, _I, C.POINTER(_P)]),
     "fkpm_dev_close": (_I, [_P]),
     "fkpm_stream": (_P, [_P]),
+    "fkpm_synchronize": (_I, [_P]),
     "fkpm_energy_scale": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _I, _P, _DP, _DP]),
     "fkpm_set_H": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _D]),
     "fkpm_set_R": (_I, [_P, _I, _I, _P]),

@@ -77,9 +77,26 @@ def build_host_lib() -> Path:
     return HOST_LIB
 
 
+CPP_TEST_SRC = ROOT / "tests" / "cpp" / "engine_flow.cpp"
+CPP_TEST_BIN = ROOT / "tests" / "cpp" / "engine_flow"
+
+
+def build_cpp_test() -> Path:
+    """The C++ application-style test of the fkpm:: API (tests/cpp)."""
+    if not CPP_TEST_SRC.exists():
+        return CPP_TEST_BIN
+    tmp = CPP_TEST_BIN.with_suffix(".tmp")
+    _run_parallel([["g++", "-O2", "-std=c++17", "-Wall", "-I", str(INCLUDE), str(CPP_TEST_SRC),
+                    "-L", str(PKG), "-lfastkpm", "-lfastkpm_hip",
+                    "-Wl,-rpath,$ORIGIN/../../fastkpm_amd", "-o", str(tmp)]])
+    os.replace(tmp, CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 def build_all() -> None:
     build_device_lib()
     build_host_lib()
+    build_cpp_test()
 
 
 if __name__ == "__main__":

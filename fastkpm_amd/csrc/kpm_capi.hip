@@ -752,6 +752,13 @@ int fkpm_dev_close(fkpm_handle* h) {
 
 void* fkpm_stream(fkpm_handle* h) { return h ? (void*)h->stream : nullptr; }
 
+int fkpm_synchronize(fkpm_handle* h) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
 int fkpm_set_H(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const int* col_idx,
                const void* val, double avg, double mag) {
   if (!h) return fail(FKPM_E_ARG, "null handle");
@@ -828,7 +835,7 @@ int fkpm_set_D_pattern(fkpm_handle* h, int nbD, int nnzbD, const int* row_ptr, c
           break;
         }
       tpos[p] = t;
-      if (t < 0) sym = false;
+      if (t < 0 && i >= j) sym = false;  // the reference aborts only for these
     }
   CHK(h->drp.ensure(sizeof(int) * (nbD + 1)));
   CHK(h->dci.ensure(sizeof(int) * (nnzbD > 0 ? nnzbD : 1)));
@@ -876,7 +883,7 @@ int fkpm_autodiff_matrix(fkpm_handle* h, int M, const double* c, void* D_val) {
                 "(held: %d)", M, h->alpha_M);
   CHK(check_D_ready(h));
   if (!h->d_sym)
-    return fail(FKPM_E_ARG, "autodiff_matrix: D pattern is not structurally symmetric");
+    return fail(FKPM_E_ARG, "autodiff_matrix: a block (i,j) with i >= j of D has no (j,i) partner");
   h->d_valid = false;
   if (h->single)
     CHK(autodiff_t<float>(h, h->kf, M, c));

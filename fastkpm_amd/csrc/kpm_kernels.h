@@ -12,14 +12,20 @@
 // Work decomposition: a wavefront is split into 64/L groups of L lanes; a
 // group owns one block row i (B orbital rows) and sweeps its Sp columns in
 // passes of L*KQ packets, each lane holding KQ 16-byte packets per orbital row.
-// Workgroups take contiguous ranges of block rows, and the workgroup index is
-// remapped so that each of the 8 XCDs (which receive workgroups round-robin)
-// owns one contiguous eighth of the rows: the gathers of a neighbour row then
-// hit that XCD's L2.
+// The neighbour blocks of a row are gathered U at a time: the column indices,
+// the U*B*KQ vector packets and the U H blocks are all requested before any
+// arithmetic, so that a wave keeps many loads in flight.  Workgroups take
+// contiguous ranges of block rows, and the workgroup index is remapped so that
+// each of the 8 XCDs (which receive workgroups round-robin) owns one
+// contiguous eighth of the rows: the gathers of a neighbour row then hit that
+// XCD's L2.
 //
-// Reductions (moment dots, D blocks) are deterministic: fixed lane butterflies,
-// fixed LDS order, per-workgroup partials reduced in index order by a second
-// kernel.  No floating-point atomics anywhere.
+// Precision: the row sums of the SpMM run in the storage precision S (FP64 or
+// FP32); moment dots, xi and D are accumulated in FP64.
+//
+// Reductions (moment dots, D blocks) are deterministic: fixed DPP lane
+// patterns, fixed LDS order, per-workgroup partials reduced in index order by a
+// second kernel.  No floating-point atomics anywhere.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,9 +45,9 @@ template <> struct Vec16<float> {
   static constexpr int N = 4;
 };
 
-// Load NSC scalars of type T (a multiple of one 16-byte vector) into doubles.
-template <typename T, int NSC>
-__device__ __forceinline__ void ldd(const T* __restrict__ p, double* v) {
+// Load NSC scalars of type T (a whole number of 16-byte vectors) as type A.
+template <typename T, int NSC, typename A>
+__device__ __forceinline__ void ldv(const T* __restrict__ p, A* v) {
   typedef typename Vec16<T>::V VT;
   constexpr int PER = Vec16<T>::N;
   static_assert(NSC % PER == 0, "load width");
@@ -49,13 +55,13 @@ __device__ __forceinline__ void ldd(const T* __restrict__ p, double* v) {
   for (int c = 0; c < NSC / PER; c++) {
     VT t = *reinterpret_cast<const VT*>(p + c * PER);
 #pragma unroll
-    for (int e = 0; e < PER; e++) v[c * PER + e] = (double)t[e];
+    for (int e = 0; e < PER; e++) v[c * PER + e] = (A)t[e];
   }
 }
 
-// Store NSC doubles as scalars of type T.
-template <typename T, int NSC>
-__device__ __forceinline__ void std_(T* __restrict__ p, const double* v) {
+// Store NSC values of type A as scalars of type T.
+template <typename T, int NSC, typename A>
+__device__ __forceinline__ void stv(T* __restrict__ p, const A* v) {
   typedef typename Vec16<T>::V VT;
   constexpr int PER = Vec16<T>::N;
   static_assert(NSC % PER == 0, "store width");
@@ -68,10 +74,6 @@ __device__ __forceinline__ void std_(T* __restrict__ p, const double* v) {
   }
 }
 
-// Round doubles to the storage type and back (the value the next step reads).
-template <typename T>
-__device__ __forceinline__ double rnd(double x) { return (double)(T)x; }
-
 // Workgroup index remapped so that XCD x (blockIdx % 8 == x) owns a
 // contiguous range of logical workgroups.
 __device__ __forceinline__ int logical_wg() {
@@ -81,10 +83,29 @@ __device__ __forceinline__ int logical_wg() {
   return x * per + (x < extra ? x : extra) + r;
 }
 
+// DPP lane move inside 16-lane rows (gfx9 encodings: quad_perm 0x00-0xFF,
+// row_mirror 0x140, row_half_mirror 0x141).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over aligned groups of L lanes, result in every lane of the group.
+// Levels 1, 2: quad_perm [1,0,3,2] and [2,3,0,1]; 4: row_half_mirror (lane
+// l <- 7-l, the other quad of the half row); 8: row_mirror (l <- 15-l, the
+// other half of the row); 16, 32: cross-row swizzles.  Every lane of a group
+// adds the same two partial sums (in either order), so all lanes agree.
 template <int L>
 __device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int m = L / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+  if constexpr (L >= 2) v += dpp_mov<0xB1>(v);
+  if constexpr (L >= 4) v += dpp_mov<0x4E>(v);
+  if constexpr (L >= 8) v += dpp_mov<0x141>(v);
+  if constexpr (L >= 16) v += dpp_mov<0x140>(v);
+  if constexpr (L >= 32) v += __shfl_xor(v, 16, WAVE);
+  if constexpr (L >= 64) v += __shfl_xor(v, 32, WAVE);
   return v;
 }
 
@@ -114,9 +135,9 @@ __device__ __forceinline__ void wg_sum2(double a, double b, double* out) {
 }
 
 // Complex (or real) multiply-accumulate on interleaved scalars:
-// acc[e] += h * x[e] for the NE elements of a packet.
-template <bool CX, int NSC>
-__device__ __forceinline__ void cmac(double* acc, double hr, double hi, const double* x) {
+// acc[e] += h * x[e] for the elements of a packet.
+template <bool CX, int NSC, typename A>
+__device__ __forceinline__ void cmac(A* acc, A hr, A hi, const A* x) {
   if constexpr (CX) {
 #pragma unroll
     for (int e = 0; e < NSC; e += 2) {
@@ -142,6 +163,8 @@ struct Geo {
   static constexpr int BBS = B * B * HS;         // scalars per H block
   static constexpr int DE = B * B;               // elements per D block
   static constexpr int DOWN = (DE + L - 1) / L;  // D elements owned per lane
+  // neighbour blocks gathered per batch (bounded by the registers they hold)
+  static constexpr int U = B <= 2 ? (KQ == 1 ? 4 : 2) : (B == 4 && KQ == 1 ? 2 : 1);
 };
 
 struct RowRange {
@@ -156,6 +179,70 @@ __device__ __forceinline__ RowRange wg_rows(int nb, int chunk) {
   return r;
 }
 
+// Block row i's gather: acc[x][k][bi] += sum_p Hs_p(bi, bj) X_x[B j_p + bj]
+// over the blocks p0..p1 of the row, for NX vectors sharing the pattern, then
+// blk(p, xv) with the gathered packets xv[x][bj][k][.] of block p.  qo is the
+// scalar offset of this lane's first packet in a row.
+template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, class F>
+__device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* __restrict__ hv,
+                                           const S* const (&X)[NX], size_t rs, size_t qo, int p0,
+                                           int p1, A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk) {
+  using GE = Geo<S, CX, B, L, KQ>;
+  // two gathered vectors hold twice the registers per block: half the batch
+  constexpr int NS = GE::NS, HS = GE::HS, BBS = GE::BBS;
+  constexpr int U = NX == 1 ? GE::U : (GE::U > 1 ? GE::U / 2 : 1);
+  for (int p = p0; p < p1; p += U) {
+    int j[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) j[u] = (p + u < p1) ? ci[p + u] : -1;
+    A xv[U][NX][B][KQ][NS];
+    A hh[U][BBS];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (j[u] >= 0) {
+#pragma unroll
+        for (int x = 0; x < NX; x++)
+#pragma unroll
+          for (int bj = 0; bj < B; bj++)
+#pragma unroll
+            for (int k = 0; k < KQ; k++)
+              ldv<S, NS>(X[x] + (size_t)(B * j[u] + bj) * rs + qo + (size_t)k * L * NS,
+                         xv[u][x][bj][k]);
+#pragma unroll
+        for (int t = 0; t < BBS; t++) hh[u][t] = (A)hv[(size_t)(p + u) * BBS + t];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (j[u] < 0) continue;
+#pragma unroll
+      for (int bj = 0; bj < B; bj++)
+#pragma unroll
+        for (int bi = 0; bi < B; bi++) {
+          const A hr = hh[u][(bj * B + bi) * HS];
+          const A hi = CX ? hh[u][(bj * B + bi) * HS + HS - 1] : A(0);
+#pragma unroll
+          for (int x = 0; x < NX; x++)
+#pragma unroll
+            for (int k = 0; k < KQ; k++) cmac<CX, NS>(acc[x][k][bi], hr, hi, xv[u][x][bj][k]);
+        }
+      blk(p + u, xv[u]);
+    }
+  }
+}
+
+// Own rows of block row i: v[k][bi] = X[B i + bi] packets of this lane.
+template <typename S, typename A, int B, int L, int KQ>
+__device__ __forceinline__ void load_rows(const S* __restrict__ X, int i, size_t rs, size_t qo,
+                                          A (&v)[KQ][B][Vec16<S>::N]) {
+  constexpr int NS = Vec16<S>::N;
+#pragma unroll
+  for (int bi = 0; bi < B; bi++)
+#pragma unroll
+    for (int k = 0; k < KQ; k++)
+      ldv<S, NS>(X + (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS, v[k][bi]);
+}
+
 // ---------------------------------------------------------------------------
 // Moments step: y = cx * (Hs x) + cz * z  (z may be null), and the two moment
 // dots sum |x_i|^2 and Re<y_i, x_i> over this workgroup's rows (i = own rows).
@@ -167,6 +254,7 @@ __global__ __launch_bounds__(WG) void k_step_moments(
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
     int chunk, double cx, double cz, double* __restrict__ partial) {
+  using A = S;
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS;
   const int lane = threadIdx.x & (WAVE - 1);
@@ -174,69 +262,47 @@ __global__ __launch_bounds__(WG) void k_step_moments(
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
   const RowRange rr = wg_rows(nb, chunk);
   const size_t rs = (size_t)qp * NS;  // scalars per orbital row
+  const A acx = (A)cx, acz = (A)cz;
   double dmm = 0.0, dm1 = 0.0;
+  const S* const X[1] = {x};
 
   for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
-    const int i = base + g;
+    int i = base + g;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     if (i >= rr.r1) continue;
     const int p0 = rp[i], p1 = rp[i + 1];
     for (int q0 = 0; q0 < qp; q0 += L * KQ) {
-      double acc[KQ][B][NS];
-      double xd[KQ][B][NS];
+      const size_t qo = (size_t)(q0 + l) * NS;
+      A xd[KQ][B][NS];
+      A acc[1][KQ][B][NS];
+      load_rows<S, A, B, L, KQ>(x, i, rs, qo, xd);
 #pragma unroll
       for (int k = 0; k < KQ; k++)
 #pragma unroll
         for (int bi = 0; bi < B; bi++)
 #pragma unroll
-          for (int e = 0; e < NS; e++) {
-            acc[k][bi][e] = 0.0;
-            xd[k][bi][e] = 0.0;
-          }
-      for (int p = p0; p < p1; p++) {
-        const int j = ci[p];
-        const S* hb = hv + (size_t)p * GE::BBS;
-        const bool diag = (j == i);
-#pragma unroll
-        for (int bj = 0; bj < B; bj++) {
-          double hc[B * GE::HS];  // block column bj (column-major block)
-#pragma unroll
-          for (int t = 0; t < B * GE::HS; t++) hc[t] = (double)hb[bj * B * GE::HS + t];
-          const S* xr = x + (size_t)(B * j + bj) * rs;
-#pragma unroll
-          for (int k = 0; k < KQ; k++) {
-            double xv[NS];
-            ldd<S, NS>(xr + (size_t)(q0 + k * L + l) * NS, xv);
-            if (diag) {
-#pragma unroll
-              for (int e = 0; e < NS; e++) xd[k][bj][e] = xv[e];
-            }
-#pragma unroll
-            for (int bi = 0; bi < B; bi++)
-              cmac<CX, NS>(acc[k][bi], hc[bi * GE::HS], CX ? hc[bi * GE::HS + GE::HS - 1] : 0.0, xv);
-          }
-        }
-      }
+          for (int e = 0; e < NS; e++) acc[0][k][bi][e] = A(0);
+      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [](int, auto&) {});
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
-        const size_t ro = (size_t)(B * i + bi) * rs;
 #pragma unroll
         for (int k = 0; k < KQ; k++) {
-          const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
-          double zv[NS];
+          const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
+          A zv[NS], out[NS];
           if (z != nullptr) {
-            ldd<S, NS>(z + off, zv);
+            ldv<S, NS>(z + off, zv);
           } else {
 #pragma unroll
-            for (int e = 0; e < NS; e++) zv[e] = 0.0;
+            for (int e = 0; e < NS; e++) zv[e] = A(0);
           }
-          double out[NS];
 #pragma unroll
-          for (int e = 0; e < NS; e++) out[e] = rnd<S>(fma(cx, acc[k][bi][e], cz * zv[e]));
-          std_<S, NS>(y + off, out);
+          for (int e = 0; e < NS; e++) out[e] = fma(acx, acc[0][k][bi][e], acz * zv[e]);
+          stv<S, NS>(y + off, out);
 #pragma unroll
           for (int e = 0; e < NS; e++) {
-            dmm = fma(xd[k][bi][e], xd[k][bi][e], dmm);
-            dm1 = fma(out[e], xd[k][bi][e], dm1);
+            const double xe = (double)xd[k][bi][e];
+            dmm = fma(xe, xe, dmm);
+            dm1 = fma((double)out[e], xe, dm1);
           }
         }
       }
@@ -256,6 +322,7 @@ __global__ __launch_bounds__(WG) void k_step_xi(
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y,
     double* __restrict__ xi, int nb, int qp, int chunk, double cx, double cz, double c0,
     double c1, int first) {
+  using A = S;
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS;
   const int lane = threadIdx.x & (WAVE - 1);
@@ -263,74 +330,51 @@ __global__ __launch_bounds__(WG) void k_step_xi(
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
   const RowRange rr = wg_rows(nb, chunk);
   const size_t rs = (size_t)qp * NS;
+  const A acx = (A)cx, acz = (A)cz;
+  const S* const X[1] = {x};
 
   for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
-    const int i = base + g;
+    int i = base + g;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     if (i >= rr.r1) continue;
     const int p0 = rp[i], p1 = rp[i + 1];
     for (int q0 = 0; q0 < qp; q0 += L * KQ) {
-      double acc[KQ][B][NS];
-      double xd[KQ][B][NS];
+      const size_t qo = (size_t)(q0 + l) * NS;
+      A acc[1][KQ][B][NS];
 #pragma unroll
       for (int k = 0; k < KQ; k++)
 #pragma unroll
         for (int bi = 0; bi < B; bi++)
 #pragma unroll
-          for (int e = 0; e < NS; e++) {
-            acc[k][bi][e] = 0.0;
-            xd[k][bi][e] = 0.0;
-          }
-      for (int p = p0; p < p1; p++) {
-        const int j = ci[p];
-        const S* hb = hv + (size_t)p * GE::BBS;
-        const bool diag = (j == i);
-#pragma unroll
-        for (int bj = 0; bj < B; bj++) {
-          double hc[B * GE::HS];  // block column bj (column-major block)
-#pragma unroll
-          for (int t = 0; t < B * GE::HS; t++) hc[t] = (double)hb[bj * B * GE::HS + t];
-          const S* xr = x + (size_t)(B * j + bj) * rs;
-#pragma unroll
-          for (int k = 0; k < KQ; k++) {
-            double xv[NS];
-            ldd<S, NS>(xr + (size_t)(q0 + k * L + l) * NS, xv);
-            if (diag) {
-#pragma unroll
-              for (int e = 0; e < NS; e++) xd[k][bj][e] = xv[e];
-            }
-#pragma unroll
-            for (int bi = 0; bi < B; bi++)
-              cmac<CX, NS>(acc[k][bi], hc[bi * GE::HS], CX ? hc[bi * GE::HS + GE::HS - 1] : 0.0, xv);
-          }
-        }
-      }
+          for (int e = 0; e < NS; e++) acc[0][k][bi][e] = A(0);
+      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [](int, auto&) {});
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
-        const size_t ro = (size_t)(B * i + bi) * rs;
 #pragma unroll
         for (int k = 0; k < KQ; k++) {
-          const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
-          double zv[NS];
+          const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
+          A zv[NS], out[NS];
           if (z != nullptr) {
-            ldd<S, NS>(z + off, zv);
+            ldv<S, NS>(z + off, zv);
           } else {
 #pragma unroll
-            for (int e = 0; e < NS; e++) zv[e] = 0.0;
+            for (int e = 0; e < NS; e++) zv[e] = A(0);
           }
-          double out[NS];
 #pragma unroll
-          for (int e = 0; e < NS; e++) out[e] = rnd<S>(fma(cx, acc[k][bi][e], cz * zv[e]));
-          std_<S, NS>(y + off, out);
+          for (int e = 0; e < NS; e++) out[e] = fma(acx, acc[0][k][bi][e], acz * zv[e]);
+          stv<S, NS>(y + off, out);
           double xa[NS];
           if (first) {
+            A xo[NS];
+            ldv<S, NS>(x + off, xo);
 #pragma unroll
-            for (int e = 0; e < NS; e++) xa[e] = fma(c0, xd[k][bi][e], c1 * out[e]);
+            for (int e = 0; e < NS; e++) xa[e] = fma(c0, (double)xo[e], c1 * (double)out[e]);
           } else {
-            ldd<double, NS>(xi + off, xa);
+            ldv<double, NS>(xi + off, xa);
 #pragma unroll
-            for (int e = 0; e < NS; e++) xa[e] = fma(c1, out[e], xa[e]);
+            for (int e = 0; e < NS; e++) xa[e] = fma(c1, (double)out[e], xa[e]);
           }
-          std_<double, NS>(xi + off, xa);
+          stv<double, NS>(xi + off, xa);
         }
       }
     }
@@ -354,6 +398,7 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
     const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
     double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
     double sb, double sb2, double dscale) {
+  using A = S;
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS;
   constexpr int HS = GE::HS;
@@ -364,157 +409,96 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
   const size_t rs = (size_t)qp * NS;
   const bool vb1 = (bm1 == nullptr);
   const double hb1 = vb1 ? sb : 1.0;  // scale of the beta_{m+1} gather
+  // a virtual beta_{m+1} gathers alpha_m a second time (one launch per call)
+  const S* const X[2] = {am, vb1 ? am : bm1};
+  const A a2 = (A)2, ab2 = (A)(2.0 * hb1), ac4 = (A)c4, ac2 = (A)c2, aw = (A)w, asb2 = (A)sb2;
+  const double fD = dscale * hb1;
 
   for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
-    const int i = base + g;
-    const bool active = i < rr.r1;
-    // Inactive groups still run the lane butterflies of FUSED_D below with
-    // zero rows (the shuffles need every lane of the wave), but load nothing.
-    const int p0 = active ? rp[i] : 0;
-    const int p1 = active ? rp[i + 1] : 0;
+    int i = base + g;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], p1 = rp[i + 1];
     for (int q0 = 0; q0 < qp; q0 += L * KQ) {
-      double ai[KQ][B][NS];
-      double acca[KQ][B][NS];
-      double accb[KQ][B][NS];
+      const size_t qo = (size_t)(q0 + l) * NS;
+      A ai[KQ][B][NS];
+      A acc[2][KQ][B][NS];
+      load_rows<S, A, B, L, KQ>(am, i, rs, qo, ai);
 #pragma unroll
-      for (int bi = 0; bi < B; bi++) {
-        const size_t ro = (size_t)(B * (active ? i : 0) + bi) * rs;
+      for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int k = 0; k < KQ; k++) {
-          if (active) {
-            ldd<S, NS>(am + ro + (size_t)(q0 + k * L + l) * NS, ai[k][bi]);
-          } else {
+        for (int k = 0; k < KQ; k++)
 #pragma unroll
-            for (int e = 0; e < NS; e++) ai[k][bi][e] = 0.0;
-          }
+          for (int bi = 0; bi < B; bi++)
 #pragma unroll
-          for (int e = 0; e < NS; e++) {
-            acca[k][bi][e] = 0.0;
-            accb[k][bi][e] = 0.0;
-          }
-        }
-      }
-      // Every group of the wave walks the same number of blocks when FUSED_D
-      // (the butterflies need all lanes); rows of different length are padded
-      // with inactive iterations.
-      int nblk = p1 - p0;
-      if constexpr (FUSED_D) {
+            for (int e = 0; e < NS; e++) acc[x][k][bi][e] = A(0);
+      gather_row<S, A, CX, B, L, KQ, 2>(
+          ci, hv, X, rs, qo, p0, p1, acc, [&](int p, auto& xv) {
+            if constexpr (FUSED_D) {
+              // D_p(bi, bj) += fD * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s])
+              double own[GE::DOWN][HS];
 #pragma unroll
-        for (int m = L; m < WAVE; m <<= 1) nblk = max(nblk, __shfl_xor(nblk, m, WAVE));
-      }
-      for (int t = 0; t < nblk; t++) {
-        const int p = p0 + t;
-        const bool live = p < p1;
-        const int j = live ? ci[p] : 0;
-        const S* hb = hv + (size_t)(live ? p : 0) * GE::BBS;
-        double xb[KQ][B][NS];
+              for (int bj = 0; bj < B; bj++)
 #pragma unroll
-        for (int bj = 0; bj < B; bj++) {
-          double hc[B * HS];  // block column bj
+                for (int bi = 0; bi < B; bi++) {
+                  A re = A(0), im = A(0);
 #pragma unroll
-          for (int u = 0; u < B * HS; u++) hc[u] = live ? (double)hb[bj * B * HS + u] : 0.0;
-          const size_t ro = (size_t)(B * j + bj) * rs;
+                  for (int k = 0; k < KQ; k++)
 #pragma unroll
-          for (int k = 0; k < KQ; k++) {
-            const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
-            double xa[NS];
-            if (live) {
-              ldd<S, NS>(am + off, xa);
-              if (vb1) {
-#pragma unroll
-                for (int e = 0; e < NS; e++) xb[k][bj][e] = xa[e];
-              } else {
-                ldd<S, NS>(bm1 + off, xb[k][bj]);
-              }
-            } else {
-#pragma unroll
-              for (int e = 0; e < NS; e++) {
-                xa[e] = 0.0;
-                xb[k][bj][e] = 0.0;
-              }
-            }
-#pragma unroll
-            for (int bi = 0; bi < B; bi++) {
-              const double hr = hc[bi * HS], hi = CX ? hc[bi * HS + HS - 1] : 0.0;
-              cmac<CX, NS>(acca[k][bi], hr, hi, xa);
-              cmac<CX, NS>(accb[k][bi], hr, hi, xb[k][bj]);
-            }
-          }
-        }
-        if constexpr (FUSED_D) {
-          // D_p(bi, bj) += dscale * hb1 * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s])
-          double own[GE::DOWN][HS];
-#pragma unroll
-          for (int o = 0; o < GE::DOWN; o++)
-#pragma unroll
-            for (int c = 0; c < HS; c++) own[o][c] = 0.0;
-#pragma unroll
-          for (int bj = 0; bj < B; bj++)
-#pragma unroll
-            for (int bi = 0; bi < B; bi++) {
-              double re = 0.0, im = 0.0;
-#pragma unroll
-              for (int k = 0; k < KQ; k++)
-#pragma unroll
-                for (int e = 0; e < NS; e += HS) {
-                  const double ar = ai[k][bi][e], br = xb[k][bj][e];
-                  re = fma(ar, br, re);
-                  if constexpr (CX) {
-                    const double aim = ai[k][bi][e + 1], bim = xb[k][bj][e + 1];
-                    re = fma(aim, bim, re);
-                    im = fma(aim, br, im);
-                    im = fma(-ar, bim, im);
+                    for (int e = 0; e < NS; e += HS) {
+                      const A ar = ai[k][bi][e], br = xv[1][bj][k][e];
+                      re = fma(ar, br, re);
+                      if constexpr (CX) {
+                        const A aim = ai[k][bi][e + 1], bim = xv[1][bj][k][e + 1];
+                        re = fma(aim, bim, re);
+                        im = fma(aim, br, im);
+                        im = fma(-ar, bim, im);
+                      }
+                    }
+                  const double sre = group_sum<L>((double)re);
+                  const double sim = CX ? group_sum<L>((double)im) : 0.0;
+                  const int de = bj * B + bi;  // column-major element index in the block
+                  if (l == de % L) {
+                    own[de / L][0] = sre;
+                    if constexpr (CX) own[de / L][HS - 1] = sim;
                   }
                 }
-              re = group_sum<L>(re);
-              if constexpr (CX) im = group_sum<L>(im);
-              const int de = bj * B + bi;  // column-major element index in the block
-              if (l == de % L) {
-                own[de / L][0] = re;
-                if constexpr (CX) own[de / L][HS - 1] = im;
+#pragma unroll
+              for (int o = 0; o < GE::DOWN; o++) {
+                const int de = o * L + l;
+                if (de < GE::DE) {
+                  double* d = dacc + ((size_t)p * GE::DE + de) * HS;
+#pragma unroll
+                  for (int c = 0; c < HS; c++) d[c] = fma(fD, own[o][c], d[c]);
+                }
               }
             }
-          if (live) {
-            const double f = dscale * hb1;
-#pragma unroll
-            for (int o = 0; o < GE::DOWN; o++) {
-              const int de = o * L + l;
-              if (de < GE::DE) {
-                double* d = dacc + ((size_t)p * GE::DE + de) * HS;
-#pragma unroll
-                for (int c = 0; c < HS; c++) d[c] = fma(f, own[o][c], d[c]);
-              }
-            }
-          }
-        }
-      }
-      if (!active) continue;
+          });
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
-        const size_t ro = (size_t)(B * i + bi) * rs;
 #pragma unroll
         for (int k = 0; k < KQ; k++) {
-          const size_t off = ro + (size_t)(q0 + k * L + l) * NS;
-          double a1[NS], an[NS], bo[NS], b2v[NS];
-          ldd<S, NS>(am1 + off, a1);
+          const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
+          A a1[NS], an[NS], bo[NS], b2v[NS];
+          ldv<S, NS>(am1 + off, a1);
 #pragma unroll
-          for (int e = 0; e < NS; e++) an[e] = rnd<S>(2.0 * acca[k][bi][e] - a1[e]);
-          std_<S, NS>(a_out + off, an);
+          for (int e = 0; e < NS; e++) an[e] = fma(a2, acc[0][k][bi][e], -a1[e]);
+          stv<S, NS>(a_out + off, an);
           if (bm2 != nullptr) {
-            ldd<S, NS>(bm2 + off, b2v);
+            ldv<S, NS>(bm2 + off, b2v);
           } else {
 #pragma unroll
-            for (int e = 0; e < NS; e++) b2v[e] = sb2 * a1[e];
+            for (int e = 0; e < NS; e++) b2v[e] = asb2 * a1[e];
           }
 #pragma unroll
           for (int e = 0; e < NS; e++) {
-            double v = fma(2.0 * hb1, accb[k][bi][e], -b2v[e]);
-            v = fma(c4, ai[k][bi][e], v);
-            v = fma(c2, a1[e], v);
-            v = fma(w, an[e], v);
-            bo[e] = rnd<S>(v);
+            A v = fma(ab2, acc[1][k][bi][e], -b2v[e]);
+            v = fma(ac4, ai[k][bi][e], v);
+            v = fma(ac2, a1[e], v);
+            v = fma(aw, an[e], v);
+            bo[e] = v;
           }
-          std_<S, NS>(b_out + off, bo);
+          stv<S, NS>(b_out + off, bo);
         }
       }
     }
@@ -579,9 +563,9 @@ __global__ __launch_bounds__(WG) void k_outer(
             double av[BI][NS], bv[B][NS];
 #pragma unroll
             for (int bi = 0; bi < BI; bi++)
-              ldd<TA, NS>(A + (size_t)(B * i + bi0 + bi) * rs + q, av[bi]);
+              ldv<TA, NS>(A + (size_t)(B * i + bi0 + bi) * rs + q, av[bi]);
 #pragma unroll
-            for (int bj = 0; bj < B; bj++) ldd<TB, NS>(Bm + (size_t)(B * j + bj) * rs + q, bv[bj]);
+            for (int bj = 0; bj < B; bj++) ldv<TB, NS>(Bm + (size_t)(B * j + bj) * rs + q, bv[bj]);
 #pragma unroll
             for (int bi = 0; bi < BI; bi++)
 #pragma unroll
@@ -704,7 +688,8 @@ __global__ void k_scale_H(const TH* __restrict__ src, S* __restrict__ dst,
 
 // D_out_k(bi, bj) = ((D_k(bi, bj) + conj(D_t(k)(bj, bi))) / 2) * inv_mag, the
 // combined SpMatBsr::symmetrize + scale(1/mag) (spmat.cpp:171-197) written out
-// of place; t(k) is the index of block (j, i).
+// of place; t(k) is the index of block (j, i), or -1 for an upper block
+// without a partner, which symmetrize leaves unchanged.
 template <bool CX>
 __global__ void k_symmetrize_scale(const double* __restrict__ din, double* __restrict__ dout,
                                    const int* __restrict__ tpos, int nnzb, int b, double inv_mag) {
@@ -716,6 +701,11 @@ __global__ void k_symmetrize_scale(const double* __restrict__ din, double* __res
     const size_t k = e / bb;
     const int r = (int)(e % bb);
     const int bj = r / b, bi = r % b;
+    if (tpos[k] < 0) {
+#pragma unroll
+      for (int h = 0; h < HS; h++) dout[e * HS + h] = din[e * HS + h] * inv_mag;
+      continue;
+    }
     const size_t e2 = (size_t)tpos[k] * bb + (size_t)bi * b + bj;
     const double re = (din[e * HS] + din[e2 * HS]) / 2.0;
     dout[e * HS] = re * inv_mag;

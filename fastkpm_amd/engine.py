@@ -139,6 +139,9 @@ class HipEngine:
     def stream(self) -> int:
         return int(lib().fkpm_stream(self._h) or 0)
 
+    def synchronize(self) -> None:
+        call("fkpm_synchronize", self._h)
+
     # -- R ------------------------------------------------------------------
     def set_R(self, R: np.ndarray) -> None:
         """Set Engine::R (n x s) and transfer it to the device (transfer_R)."""
@@ -146,6 +149,22 @@ class HipEngine:
         n, s = R.shape
         self.R = R
         call("fkpm_set_R", self._h, n, s, R.ctypes.data_as(C.c_void_p))
+
+    def set_R_identity(self, n: int, j_start: int = 0, j_end: int | None = None) -> None:
+        from . import host
+        self.set_R(host.set_R_identity(self.dtype, n, j_start, j_end))
+
+    def set_R_uncorrelated(self, n: int, s: int, rng, j_start: int = 0,
+                           j_end: int | None = None) -> None:
+        from . import host
+        self.set_R(host.set_R_uncorrelated(self.dtype, n, s, rng, j_start, j_end))
+
+    def set_R_correlated(self, groups, rng, j_start: int | None = None,
+                         j_end: int | None = None) -> None:
+        """Engine::set_R_correlated (src/engine.cpp:76-116), drawing from the
+        C++ library's std::mt19937 `rng` (fastkpm_amd.host.RNG)."""
+        from . import host
+        self.set_R(host.set_R_correlated(self.dtype, groups, rng, j_start, j_end))
 
     # -- spectrum -----------------------------------------------------------
     def energy_scale(self, H: SpMatBsr, extend: float, iters: int = 128,

@@ -70,6 +70,9 @@ int fkpm_dev_close(fkpm_handle* h);
 /* The HIP stream (hipStream_t) all work of this handle is queued on. */
 void* fkpm_stream(fkpm_handle* h);
 
+/* Wait until all work queued on the handle's stream has finished. */
+int fkpm_synchronize(fkpm_handle* h);
+
 /* Lanczos estimate of the spectral bounds of the unscaled H, FP64 on the
  * device; the iters x iters tridiagonal eigenproblem is solved on the host.
  * Replaces Engine_cuSPARSE::energy_scale (engine_cusparse.cpp:598-673).

@@ -1,0 +1,57 @@
+"""The C++ fkpm:: API (include/fastkpm.h) driven by an application-style
+program, tests/cpp/engine_flow.cpp: host checks on the CPU, and on the GPU the
+full flow compared with the oracle on the H and R the program used."""
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+BIN = Path(__file__).resolve().parent / "cpp" / "engine_flow"
+
+
+def _bin():
+    if not BIN.exists():
+        pytest.fail(f"{BIN} is not built (run __graft_entry__.build())")
+    return str(BIN)
+
+
+def test_cpp_api_host_checks():
+    r = subprocess.run([_bin(), "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "host OK" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_api_device_flow_matches_oracle(oracle, tmp_path):
+    r = subprocess.run([_bin(), "device", str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from fastkpm_amd import SpMatBsr
+    meta = (tmp_path / "meta.txt").read_text().split()
+    n, s, M, nb, nnzb = (int(v) for v in meta[:5])
+    lo, hi, E = (float(v) for v in meta[5:8])
+    rd = lambda name, dt: np.fromfile(tmp_path / name, dtype=dt)
+    H = SpMatBsr(nb, nb, 2, rd("H_row_idx.bin", np.int32), rd("H_col_idx.bin", np.int32),
+                 rd("H_row_ptr.bin", np.int32), rd("H_val.bin", np.complex128))
+    assert H.n_blocks() == nnzb
+    R = rd("R.bin", np.complex128).reshape(s, n).T
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    mu_o = orc.moments(M)
+    assert rel_err(rd("mu.bin", np.float64), mu_o) < 1e-10
+    cE = rd("cE.bin", np.float64)
+    assert E == pytest.approx(float(np.dot(cE, mu_o)), rel=1e-10)
+    d_o = orc.autodiff_matrix(cE, H)
+    assert rel_err(rd("D_autodiff.bin", np.complex128), d_o) < 1e-10
+    s_o = orc.stoch_matrix(rd("cF.bin", np.float64), H)
+    assert rel_err(rd("D_stoch.bin", np.complex128), s_o) < 1e-10
+    # the program drew R through the product's set_R_correlated; the oracle's
+    # restatement of engine.cpp draws the same phases
+    groups = np.argmax(np.abs(R) > 0, axis=1)
+    assert np.array_equal(oracle.set_R_correlated(groups, oracle.MT19937(2024)), R)
