@@ -13,6 +13,9 @@ LIB_PATH = _PKG / "libfastkpm_hip.so"
 
 F32, F64, C32, C64 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
+OPT_STOCH_METHOD, OPT_AUTOTUNE = 1, 2
+STOCH_CLENSHAW, STOCH_FORWARD = 0, 1
+KERNEL_MOMENTS, KERNEL_XI, KERNEL_AUTODIFF, KERNEL_OUTER = 0, 1, 2, 3
 
 ERRORS = {-1: "FKPM_E_ARG", -2: "FKPM_E_STATE", -3: "FKPM_E_HIP", -4: "FKPM_E_NODEV",
           -5: "FKPM_E_UNSUP"}
@@ -39,6 +42,11 @@ SIGNATURES = {
     "fkpm_get_timing": (_I, [_P, _DP, _IP, _DP, _IP, _DP, _IP]),
     "fkpm_get_config": (_I, [_P, _IP, _IP, _IP, _IP, _IP]),
     "fkpm_set_geometry": (_I, [_P, _I, _I, _I]),
+    "fkpm_bandwidth_probe": (_I, [_P, C.c_size_t, _I, _DP]),
+    "fkpm_set_option": (_I, [_P, _I, _I]),
+    "fkpm_autotune": (_I, [_P]),
+    "fkpm_get_kernel_config": (_I, [_P, _I, _IP, _IP, _IP, _IP]),
+    "fkpm_get_state": (_I, [_P, _IP, _IP, _IP]),
 }
 
 _lib = None

@@ -8,10 +8,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -135,6 +139,26 @@ int check_pattern(const char* what, int nb, int nnzb, const int* row_ptr, const 
   return 0;
 }
 
+// Kernel kinds with a geometry of their own.
+enum KKind { K_MOM = 0, K_XI = 1, K_AD = 2, K_OUT = 3, NKIND = 4 };
+
+// Geometry of one kernel kind: lanes per block row, packets per lane,
+// workgroup cap and nontemporal bits; nwg/chunk derived from them.
+struct KCfg {
+  int L = 0, KQ = 0, maxwg = 0, nt = 0;
+  int nwg = 0, chunk = 0;
+};
+
+template <class S>
+fkd::RSource<S> no_rsource() {
+  fkd::RSource<S> r;
+  r.rd = nullptr;
+  r.rcol = nullptr;
+  r.rval = nullptr;
+  r.cr = 0.0;
+  return r;
+}
+
 }  // namespace
 
 struct fkpm_handle {
@@ -155,10 +179,18 @@ struct fkpm_handle {
   // --- R and recursion state ---
   int n = 0, s = 0;
   bool r_set = false, r_dirty = true;
-  DevBuf rraw;  // R as given (host layout and type)
-  int L = 0, KQ = 0, qp = 0, sp = 0, nwg = 0, chunk = 0, geo_b = 0;
+  DevBuf rraw;             // R as given (host layout and type)
+  bool r_compact = false;  // at most one nonzero per row (probing colours)
+  DevBuf rcol, rval, rflag;
+  int qp = 0, sp = 0, wmax = 0, geo_b = 0;
+  KCfg kc[NKIND];
+  fkd::KSet<double> kd[NKIND];
+  fkd::KSet<float> kf[NKIND];
   int force_L = 0, force_KQ = 0, max_wg = 0;
-  DevBuf R, A[3], Bv[3], xi, partial, psum;
+  bool tuned = false;
+  bool autotune_on_use = false;
+  int stoch_method = FKPM_STOCH_CLENSHAW;
+  DevBuf R, A[3], Bv[3], xi, partial, psum, tune_dacc;
   int alpha_M = 0;  // M of the moments whose final state is held, 0 if none
 
   // --- D ---
@@ -173,14 +205,13 @@ struct fkpm_handle {
   double ms_mom = 0, ms_stoch = 0, ms_ad = 0;
   int n_mom = 0, n_stoch = 0, n_ad = 0;
 
-  fkd::KSet<double> kd;
-  fkd::KSet<float> kf;
-
   ~fkpm_handle() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    DevBuf* bufs[] = {&rp, &ci, &hv, &rowblk, &hraw, &rraw, &R, &A[0], &A[1], &A[2], &Bv[0],
-                      &Bv[1], &Bv[2], &xi, &partial, &psum, &drp, &dci, &tpos, &dacc, &dout};
+    DevBuf* bufs[] = {&rp,    &ci,    &hv,      &rowblk, &hraw, &rraw,  &rcol,     &rval,
+                      &rflag, &R,     &A[0],    &A[1],   &A[2], &Bv[0], &Bv[1],    &Bv[2],
+                      &xi,    &partial, &psum,  &tune_dacc, &drp, &dci,  &tpos,     &dacc,
+                      &dout};
     for (DevBuf* d : bufs) d->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -191,76 +222,168 @@ struct fkpm_handle {
   size_t accbytes() const { return (size_t)n * sp * (cx ? 2 : 1) * 8; }
   int ne() const { return (single ? 4 : 2) / (cx ? 2 : 1); }  // elements per 16-byte packet
   void* vec_a(int k) { return k == 0 ? R.p : A[k % 3].p; }  // forward-phase alpha_k
+  template <class S>
+  const fkd::KSet<S>& ks(int kind) const;
 };
+
+template <>
+const fkd::KSet<double>& fkpm_handle::ks<double>(int kind) const {
+  return kd[kind];
+}
+template <>
+const fkd::KSet<float>& fkpm_handle::ks<float>(int kind) const {
+  return kf[kind];
+}
 
 namespace {
 
-int find_kset(fkpm_handle* h, int B, int L, int KQ) {
-  bool ok;
+bool lookup_kset(fkpm_handle* h, int kind, int B, int L, int KQ) {
   if (h->single)
-    ok = h->cx ? fkd::find_kset_fc(B, L, KQ, &h->kf) : fkd::find_kset_fr(B, L, KQ, &h->kf);
-  else
-    ok = h->cx ? fkd::find_kset_dc(B, L, KQ, &h->kd) : fkd::find_kset_dr(B, L, KQ, &h->kd);
-  if (!ok) return fail(FKPM_E_UNSUP, "no kernel for block %d, lanes %d, packets %d", B, L, KQ);
+    return h->cx ? fkd::find_kset_fc(B, L, KQ, &h->kf[kind]) : fkd::find_kset_fr(B, L, KQ, &h->kf[kind]);
+  return h->cx ? fkd::find_kset_dc(B, L, KQ, &h->kd[kind]) : fkd::find_kset_dr(B, L, KQ, &h->kd[kind]);
+}
+
+bool compiled(int b, int L, int KQ) {
+  fkd::GeomChoice gs[64];
+  const int ng = fkd::geometries_for_block(b, gs, 64);
+  for (int k = 0; k < ng; k++)
+    if (gs[k].L == L && gs[k].KQ == KQ) return true;
+  return false;
+}
+
+void derive_grid(int nb, KCfg& c) {
+  const int gwg = (fkd::WG / fkd::WAVE) * (fkd::WAVE / c.L);
+  const int cap = c.maxwg > 0 ? c.maxwg : 4096;
+  int nwg = (nb + gwg - 1) / gwg;
+  if (nwg > cap) nwg = cap;
+  if (nwg < 1) nwg = 1;
+  int chunk = (nb + nwg - 1) / nwg;
+  chunk = ((chunk + gwg - 1) / gwg) * gwg;
+  c.nwg = (nb + chunk - 1) / chunk;
+  c.chunk = chunk;
+}
+
+int set_kind(fkpm_handle* h, int kind, KCfg c) {
+  if (!lookup_kset(h, kind, h->b, c.L, c.KQ))
+    return fail(FKPM_E_UNSUP, "no kernel for block %d, lanes %d, packets %d", h->b, c.L, c.KQ);
+  derive_grid(h->nb, c);
+  h->kc[kind] = c;
   return 0;
 }
 
-// Choose (L, KQ): the smallest compiled L*KQ that covers the row of packets in
-// one pass (the fused D reduction then runs once per block), preferring more
-// packets per lane; a forced geometry wins.
+// Default geometry of a kernel kind when nothing is tuned: one pass of width
+// wmax; the single-gather kinds take the widest lane groups (wave-uniform
+// rows), the kinds with block reductions groups of 16 lanes (one DPP row).
+KCfg default_cfg(fkpm_handle* h, int kind) {
+  static const int pref_gather[] = {64, 32, 16, 8, 4};
+  static const int pref_reduce[] = {16, 8, 32, 4, 64};
+  const int* pref = (kind == K_MOM || kind == K_XI) ? pref_gather : pref_reduce;
+  KCfg c;
+  for (int t = 0; t < 5 && c.L == 0; t++) {
+    const int L = pref[t];
+    if (h->wmax % L == 0 && compiled(h->b, L, h->wmax / L)) {
+      c.L = L;
+      c.KQ = h->wmax / L;
+    }
+  }
+  if (c.L == 0) {  // any compiled geometry of width <= wmax, widest first
+    fkd::GeomChoice gs[64];
+    const int ng = fkd::geometries_for_block(h->b, gs, 64);
+    for (int k = 0; k < ng; k++) {
+      const int w = gs[k].L * gs[k].KQ;
+      if (w <= h->wmax && (c.L == 0 || w > c.L * c.KQ)) {
+        c.L = gs[k].L;
+        c.KQ = gs[k].KQ;
+      }
+    }
+  }
+  c.maxwg = 4096;
+  c.nt = 0;
+  return c;
+}
+
+// Process-wide cache of tuned geometries, keyed by the problem shape.
+std::mutex g_tune_mu;
+std::map<std::string, std::array<KCfg, NKIND>> g_tune_cache;
+
+std::string tune_key(const fkpm_handle* h) {
+  char k[128];
+  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb, h->nnzb);
+  return k;
+}
+
+// Packed width: qp packets per row, a multiple of wmax, the smallest compiled
+// pass width that covers the row (or the widest, with several passes).
 int choose_geometry(fkpm_handle* h) {
   const int ne = h->ne();
   const int qp0 = (h->s + ne - 1) / ne;
-  fkd::GeomChoice gs[32];
-  const int ng = fkd::geometries_for_block(h->b, gs, 32);
+  fkd::GeomChoice gs[64];
+  const int ng = fkd::geometries_for_block(h->b, gs, 64);
   if (ng == 0) return fail(FKPM_E_UNSUP, "block size %d is not supported", h->b);
-  int L = 0, KQ = 0;
-  if (h->force_L > 0) {
-    L = h->force_L;
-    KQ = h->force_KQ > 0 ? h->force_KQ : 1;
-  } else {
-    int best = -1;
-    for (int k = 0; k < ng; k++) {
-      const int w = gs[k].L * gs[k].KQ;
-      if (w >= qp0) {
-        if (best < 0 || w < gs[best].L * gs[best].KQ ||
-            (w == gs[best].L * gs[best].KQ && gs[k].KQ > gs[best].KQ))
-          best = k;
-      }
-    }
-    if (best < 0) {  // several passes: widest geometry
-      best = 0;
-      for (int k = 1; k < ng; k++)
-        if (gs[k].L * gs[k].KQ > gs[best].L * gs[best].KQ) best = k;
-    }
-    L = gs[best].L;
-    KQ = gs[best].KQ;
+  int wmax = 0, widest = 0;
+  for (int k = 0; k < ng; k++) {
+    const int w = gs[k].L * gs[k].KQ;
+    widest = std::max(widest, w);
+    if (w >= qp0 && (wmax == 0 || w < wmax)) wmax = w;
   }
-  CHK(find_kset(h, h->b, L, KQ));
-  const int w = L * KQ;
-  h->L = L;
-  h->KQ = KQ;
-  h->qp = ((qp0 + w - 1) / w) * w;
+  if (wmax == 0) wmax = widest;
+  if (h->force_L > 0) {
+    const int fkq = h->force_KQ > 0 ? h->force_KQ : 1;
+    if (!compiled(h->b, h->force_L, fkq))
+      return fail(FKPM_E_UNSUP, "geometry lanes=%d kq=%d not compiled for block %d", h->force_L,
+                  fkq, h->b);
+    wmax = std::max(wmax, h->force_L * fkq);
+  }
+  h->wmax = wmax;
+  h->qp = ((qp0 + wmax - 1) / wmax) * wmax;
   h->sp = h->qp * ne;
-  const int gwg = (fkd::WG / fkd::WAVE) * (fkd::WAVE / L);
-  const int cap = h->max_wg > 0 ? h->max_wg : 2048;
-  int nwg = (h->nb + gwg - 1) / gwg;
-  if (nwg > cap) nwg = cap;
-  if (nwg < 1) nwg = 1;
-  int chunk = (h->nb + nwg - 1) / nwg;
-  chunk = ((chunk + gwg - 1) / gwg) * gwg;
-  nwg = (h->nb + chunk - 1) / chunk;
-  h->nwg = nwg;
-  h->chunk = chunk;
   h->geo_b = h->b;
+  bool cached = false;
+  std::array<KCfg, NKIND> cc;
+  if (h->force_L == 0) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(tune_key(h));
+    if (it != g_tune_cache.end()) {
+      cc = it->second;
+      cached = true;
+    }
+  }
+  for (int kind = 0; kind < NKIND; kind++) {
+    KCfg c;
+    if (h->force_L > 0) {
+      c.L = h->force_L;
+      c.KQ = h->force_KQ > 0 ? h->force_KQ : 1;
+      c.maxwg = h->max_wg;
+    } else if (cached) {
+      c = cc[kind];
+    } else {
+      c = default_cfg(h, kind);
+      if (h->max_wg > 0) c.maxwg = h->max_wg;
+    }
+    CHK(set_kind(h, kind, c));
+  }
+  h->tuned = cached;
   return 0;
 }
 
 template <class TH, class S, bool CX>
 int pack_R_t(fkpm_handle* h) {
   dim3 grid((h->n + 63) / 64, (h->sp + 15) / 16);
-  return launch(&fkd::k_pack_R<TH, S, CX>, grid, dim3(fkd::WG), h->stream,
-                h->rraw.as<const TH>(), h->R.as<S>(), h->n, h->s, h->sp);
+  CHK(launch(&fkd::k_pack_R<TH, S, CX>, grid, dim3(fkd::WG), h->stream, h->rraw.as<const TH>(),
+             h->R.as<S>(), h->n, h->s, h->sp));
+  // compact form when every row holds at most one nonzero
+  CHK(h->rcol.ensure(sizeof(int) * (size_t)h->n));
+  CHK(h->rval.ensure(sizeof(S) * (size_t)h->n * (CX ? 2 : 1)));
+  CHK(h->rflag.ensure(16));
+  CHK(launch(&fkd::k_zero, dim3(1), dim3(64), h->stream, (double*)h->rflag.p, (size_t)2));
+  CHK(launch(&fkd::k_compact_R<TH, S, CX>, dim3((h->n + fkd::WG - 1) / fkd::WG), dim3(fkd::WG),
+             h->stream, h->rraw.as<const TH>(), h->n, h->s, h->rcol.as<int>(), h->rval.as<S>(),
+             h->rflag.as<int>()));
+  int flag = 1;
+  HIPCHK(hipMemcpyAsync(&flag, h->rflag.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->r_compact = (flag == 0);
+  return 0;
 }
 
 template <class S>
@@ -272,6 +395,21 @@ int pack_R_s(fkpm_handle* h) {
     default: return pack_R_t<double, S, true>(h);
   }
 }
+
+template <class S>
+fkd::RSource<S> rsource(fkpm_handle* h, double cr) {
+  fkd::RSource<S> r = no_rsource<S>();
+  r.cr = cr;
+  if (h->r_compact) {
+    r.rcol = h->rcol.as<const int>();
+    r.rval = h->rval.as<const S>();
+  } else {
+    r.rd = h->R.as<const S>();
+  }
+  return r;
+}
+
+int tune(fkpm_handle* h);
 
 // Make geometry, vectors and packed R consistent with the current H and R.
 int ensure_ready(fkpm_handle* h) {
@@ -289,15 +427,15 @@ int ensure_ready(fkpm_handle* h) {
     CHK(pack_R_s<float>(h));
   else
     CHK(pack_R_s<double>(h));
-  // A-buffers: padded columns must be zero; every element is written by the
-  // recursion before it is read, so no clear is needed.
+  // A-buffers: every element is written by the recursion before it is read.
   h->r_dirty = false;
   h->alpha_M = 0;
+  if (h->autotune_on_use && !h->tuned && h->force_L == 0) CHK(tune(h));
   return 0;
 }
 
 int ensure_partials(fkpm_handle* h, int nsteps) {
-  CHK(h->partial.ensure((size_t)nsteps * h->nwg * 2 * sizeof(double)));
+  CHK(h->partial.ensure((size_t)nsteps * h->kc[K_MOM].nwg * 2 * sizeof(double)));
   CHK(h->psum.ensure((size_t)nsteps * 2 * sizeof(double)));
   return 0;
 }
@@ -322,13 +460,130 @@ int timing_end(fkpm_handle* h, double* ms, int* cnt, int launches) {
 }
 
 // ---------------------------------------------------------------------------
+// Autotuner: times every compiled geometry of pass width wmax or wmax/2, three
+// workgroup caps and the nontemporal choice, per kernel kind, on the current
+// H and R; keeps the fastest and caches it for this problem shape.
+// ---------------------------------------------------------------------------
+template <class S>
+int tune_t(fkpm_handle* h) {
+  const int HS = h->cx ? 2 : 1;
+  const size_t vb = h->vbytes();
+  CHK(h->Bv[0].ensure(vb));
+  CHK(h->xi.ensure(h->accbytes()));
+  CHK(h->tune_dacc.ensure((size_t)(h->nnzb > 0 ? h->nnzb : 1) * h->b * h->b * HS * sizeof(double)));
+  CHK(h->partial.ensure((size_t)(h->nb + 3) / 4 * 2 * sizeof(double) + 64));
+  const int* rp = h->rp.as<const int>();
+  const int* ci = h->ci.as<const int>();
+  const S* hv = h->hv.as<const S>();
+  const S* R = h->R.as<const S>();
+  S* A0 = h->A[0].as<S>();
+  S* A1 = h->A[1].as<S>();
+  S* A2 = h->A[2].as<S>();
+  S* B0 = h->Bv[0].as<S>();
+  CHK(h->Bv[1].ensure(vb));
+  CHK(h->Bv[2].ensure(vb));
+  S* B1 = h->Bv[1].as<S>();
+  S* B2 = h->Bv[2].as<S>();
+  // finite inputs for every candidate: every ring buffer = Hs R
+  {
+    const KCfg& c = h->kc[K_MOM];
+    const auto& ks = h->ks<S>(K_MOM);
+    S* outs[] = {A0, A1, A2, B0, B1, B2};
+    for (S* o : outs)
+      CHK(launch(ks.mom, dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R, (const S*)nullptr,
+                 o, h->nb, h->qp, c.chunk, 1.0, 0.0, (double*)nullptr, 0, no_rsource<S>()));
+    CHK(zero_doubles(h, h->tune_dacc.p, (size_t)h->nnzb * h->b * h->b * HS));
+  }
+  S* ring[3] = {A0, A1, A2};
+  S* bring[3] = {B0, B1, B2};
+  const bool log = getenv("FKPM_TUNE_LOG") != nullptr;
+  int step = 0;
+  fkd::GeomChoice gs[64];
+  const int ng = fkd::geometries_for_block(h->b, gs, 64);
+  static const int caps[] = {1024, 4096, 16384};
+  static const int nts[] = {0, fkd::NT_LOAD | fkd::NT_STORE};
+  const int kinds[] = {K_MOM, K_XI, K_AD};
+  for (int kind : kinds) {
+    KCfg best = h->kc[kind];
+    float best_ms = 1e30f;
+    for (int g = 0; g < ng; g++) {
+      const int w = gs[g].L * gs[g].KQ;
+      if (w > h->wmax || 2 * w < h->wmax) continue;
+      for (int cap : caps)
+        for (int nt : nts) {
+          KCfg c;
+          c.L = gs[g].L;
+          c.KQ = gs[g].KQ;
+          c.maxwg = cap;
+          c.nt = nt;
+          CHK(set_kind(h, kind, c));
+          const KCfg& k = h->kc[kind];
+          const auto& ks = h->ks<S>(kind);
+          // steps rotate their buffers as the recursions do (one continuous
+          // recurrence over all candidates keeps the values bounded)
+          auto run = [&]() -> int {
+            const int t = step++;
+            S* x = ring[(t + 1) % 3];
+            S* z = ring[t % 3];
+            S* y = ring[(t + 2) % 3];
+            // recurrences keep |x| bounded: y = 2 Hs x - z stays within the spectrum scale
+            if (kind == K_MOM)
+              return launch(ks.mom, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+                            (const S*)x, (const S*)z, y, h->nb, h->qp, k.chunk, 2.0, -1.0,
+                            h->partial.as<double>(), k.nt, no_rsource<S>());
+            if (kind == K_XI)
+              return launch(ks.xi, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, (const S*)x,
+                            (const S*)z, y, h->xi.as<double>(), h->nb, h->qp, k.chunk, 2.0, -1.0,
+                            0.0, 0.5, t == 0 ? 1 : 0, k.nt);
+            // backward step: reads alpha_m, alpha_{m+1}, beta_{m+1}, beta_{m+2};
+            // writes alpha_{m-1}, beta_m into the third buffers
+            return launch(ks.ad_fused, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+                          (const S*)x, (const S*)z, (const S*)bring[(t + 1) % 3],
+                          (const S*)bring[t % 3], y, bring[(t + 2) % 3], h->tune_dacc.as<double>(),
+                          h->nb, h->qp, k.chunk, 0.1, 0.1, 0.1, 1.0, 0.0, 2.0, k.nt);
+          };
+          CHK(run());
+          HIPCHK(hipEventRecord(h->ev0, h->stream));
+          for (int r = 0; r < 6; r++) CHK(run());
+          HIPCHK(hipEventRecord(h->ev1, h->stream));
+          HIPCHK(hipEventSynchronize(h->ev1));
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+          if (log)
+            fprintf(stderr, "[fkpm tune] kind=%d L=%d KQ=%d maxwg=%d nwg=%d nt=%d: %.2f us\n", kind,
+                    k.L, k.KQ, k.maxwg, k.nwg, k.nt, 1e3 * ms / 6);
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = k;
+          }
+        }
+    }
+    CHK(set_kind(h, kind, best));
+  }
+  KCfg outc = h->kc[K_AD];  // outer products: once per call, the autodiff geometry
+  outc.nt = 0;
+  CHK(set_kind(h, K_OUT, outc));
+  h->alpha_M = 0;
+  h->tuned = true;
+  std::array<KCfg, NKIND> cc;
+  for (int k = 0; k < NKIND; k++) cc[k] = h->kc[k];
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[tune_key(h)] = cc;
+  return 0;
+}
+
+int tune(fkpm_handle* h) { return h->single ? tune_t<float>(h) : tune_t<double>(h); }
+
+// ---------------------------------------------------------------------------
 // moments (engine_cpu.cpp:63-84; engine_cusparse.cpp:755-791)
 // ---------------------------------------------------------------------------
 template <class S>
-int moments_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, double* mu) {
+int moments_t(fkpm_handle* h, int M, double* mu) {
   const int nsteps = M / 2;
   CHK(ensure_partials(h, nsteps));
-  const dim3 grid(h->nwg), block(fkd::WG);
+  const KCfg& kc = h->kc[K_MOM];
+  const auto& ks = h->ks<S>(K_MOM);
+  const dim3 grid(kc.nwg), block(fkd::WG);
   CHK(timing_begin(h));
   for (int m = 0; m < nsteps; m++) {
     // alpha_{m+1} = cx Hs alpha_m + cz alpha_{m-1}
@@ -336,12 +591,13 @@ int moments_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, double* mu) {
     const S* z = m == 0 ? nullptr : (const S*)h->vec_a(m - 1);
     S* y = (S*)h->vec_a(m + 1);
     CHK(launch(ks.mom, grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
-               h->hv.as<const S>(), x, z, y, h->nb, h->qp, h->chunk, m == 0 ? 1.0 : 2.0,
-               m == 0 ? 0.0 : -1.0, h->partial.as<double>() + (size_t)m * h->nwg * 2));
+               h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk, m == 0 ? 1.0 : 2.0,
+               m == 0 ? 0.0 : -1.0, h->partial.as<double>() + (size_t)m * kc.nwg * 2,
+               kc.nt, no_rsource<S>()));
   }
   CHK(timing_end(h, &h->ms_mom, &h->n_mom, nsteps));
   CHK(launch(&fkd::k_reduce_partials, dim3(nsteps), block, h->stream,
-             (const double*)h->partial.as<double>(), h->nwg, h->psum.as<double>()));
+             (const double*)h->partial.as<double>(), kc.nwg, h->psum.as<double>()));
   std::vector<double> ps((size_t)nsteps * 2);
   HIPCHK(hipMemcpyAsync(ps.data(), h->psum.p, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
                         h->stream));
@@ -388,34 +644,92 @@ int check_D_ready(fkpm_handle* h) {
 // ---------------------------------------------------------------------------
 // stoch_matrix (engine_cpu.cpp:229-253; engine_cusparse.cpp:910-953)
 // ---------------------------------------------------------------------------
+
+// xi = sum_{m<M} c_m T_m(Hs) R by the Clenshaw recurrence
+//   b_{M-1} = c_{M-1} R, b_M = 0,
+//   b_k = c_k R + 2 Hs b_{k+1} - b_{k+2}   (k = M-2 .. 1),
+//   xi = c_0 R + Hs b_1 - b_2,
+// one SpMM pass per step reading two vectors (no FP64 accumulator traffic);
+// b_{M-1} stays virtual (a multiple of R).  Returns xi's buffer.
 template <class S>
-int stoch_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
-  CHK(h->xi.ensure(h->accbytes()));
-  const dim3 grid(h->nwg), block(fkd::WG);
+int stoch_clenshaw(fkpm_handle* h, int M, const double* c, const S** xi_out) {
+  const KCfg& kc = h->kc[K_MOM];
+  const auto& ks = h->ks<S>(K_MOM);
+  const dim3 grid(kc.nwg), block(fkd::WG);
+  const S* R = (const S*)h->R.p;
+  struct Vb {
+    const S* p;
+    double s;
+  };
+  Vb b1{R, c[M - 1]}, b2{nullptr, 0.0};
+  int slot = 0, launches = 0;
+  auto step = [&](double xfac, double cr, S* y) -> int {
+    return launch(ks.mom, grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
+                  h->hv.as<const S>(), b1.p, b2.p, y, h->nb, h->qp, kc.chunk, xfac * b1.s,
+                  b2.p ? -b2.s : 0.0, (double*)nullptr, kc.nt, rsource<S>(h, cr));
+  };
+  for (int k = M - 2; k >= 1; k--) {
+    S* y = (S*)h->A[slot].p;
+    slot = (slot + 1) % 3;
+    CHK(step(2.0, c[k], y));
+    launches++;
+    b2 = b1;
+    b1 = Vb{y, 1.0};
+  }
+  S* y = (S*)h->A[slot].p;
+  CHK(step(1.0, c[0], y));
+  *xi_out = y;
+  return launches + 1;
+}
+
+template <class S>
+int stoch_t(fkpm_handle* h, int M, const double* c) {
+  const dim3 block(fkd::WG);
   const int* rp = h->rp.as<const int>();
   const int* ci = h->ci.as<const int>();
   const S* hv = h->hv.as<const S>();
-  CHK(timing_begin(h));
-  // alpha_1 = Hs R, xi = c0 R + c1 alpha_1
-  CHK(launch(ks.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(0), (const S*)nullptr,
-             (S*)h->vec_a(1), h->xi.as<double>(), h->nb, h->qp, h->chunk, 1.0, 0.0, c[0], c[1],
-             1));
-  for (int m = 2; m < M; m++) {
-    // alpha_m = 2 Hs alpha_{m-1} - alpha_{m-2}; xi += c_m alpha_m
-    CHK(launch(ks.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(m - 1),
-               (const S*)h->vec_a(m - 2), (S*)h->vec_a(m), h->xi.as<double>(), h->nb, h->qp,
-               h->chunk, 2.0, -1.0, 0.0, c[m], 0));
-  }
-  CHK(timing_end(h, &h->ms_stoch, &h->n_stoch, M - 1));
   const size_t dcount = (size_t)h->nnzbD * h->b * h->b * (h->cx ? 2 : 1);
-  CHK(zero_doubles(h, h->dacc.p, dcount));
-  // D = (1/2) R xi^+ + (1/2) xi R^+
-  CHK(launch(ks.outer_sd, grid, block, h->stream, h->drp.as<const int>(), h->dci.as<const int>(),
-             (const S*)h->vec_a(0), (const double*)h->xi.as<double>(), h->dacc.as<double>(), h->nb,
-             h->qp, h->chunk, 0.5));
-  CHK(launch(ks.outer_ds, grid, block, h->stream, h->drp.as<const int>(), h->dci.as<const int>(),
-             (const double*)h->xi.as<double>(), (const S*)h->vec_a(0), h->dacc.as<double>(), h->nb,
-             h->qp, h->chunk, 0.5));
+  const KCfg& ko = h->kc[K_OUT];
+  const auto& kso = h->ks<S>(K_OUT);
+  CHK(timing_begin(h));
+  if (h->stoch_method == FKPM_STOCH_CLENSHAW) {
+    const S* xi = nullptr;
+    const int launches = stoch_clenshaw<S>(h, M, c, &xi);
+    if (launches < 0) return launches;
+    CHK(timing_end(h, &h->ms_stoch, &h->n_stoch, launches));
+    CHK(zero_doubles(h, h->dacc.p, dcount));
+    // D = (1/2) R xi^+ + (1/2) xi R^+
+    CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, h->drp.as<const int>(),
+               h->dci.as<const int>(), (const S*)h->R.p, xi, h->dacc.as<double>(), h->nb, h->qp,
+               ko.chunk, 0.5));
+    CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, h->drp.as<const int>(),
+               h->dci.as<const int>(), xi, (const S*)h->R.p, h->dacc.as<double>(), h->nb, h->qp,
+               ko.chunk, 0.5));
+  } else {
+    // the reference's order: forward recursion, xi accumulated in FP64
+    CHK(h->xi.ensure(h->accbytes()));
+    const KCfg& kx = h->kc[K_XI];
+    const auto& ksx = h->ks<S>(K_XI);
+    const dim3 grid(kx.nwg);
+    // alpha_1 = Hs R, xi = c0 R + c1 alpha_1
+    CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(0),
+               (const S*)nullptr, (S*)h->vec_a(1), h->xi.as<double>(), h->nb, h->qp, kx.chunk, 1.0,
+               0.0, c[0], c[1], 1, kx.nt));
+    for (int m = 2; m < M; m++) {
+      // alpha_m = 2 Hs alpha_{m-1} - alpha_{m-2}; xi += c_m alpha_m
+      CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(m - 1),
+                 (const S*)h->vec_a(m - 2), (S*)h->vec_a(m), h->xi.as<double>(), h->nb, h->qp,
+                 kx.chunk, 2.0, -1.0, 0.0, c[m], 0, kx.nt));
+    }
+    CHK(timing_end(h, &h->ms_stoch, &h->n_stoch, M - 1));
+    CHK(zero_doubles(h, h->dacc.p, dcount));
+    CHK(launch(kso.outer_sd, dim3(ko.nwg), block, h->stream, h->drp.as<const int>(),
+               h->dci.as<const int>(), (const S*)h->vec_a(0), (const double*)h->xi.as<double>(),
+               h->dacc.as<double>(), h->nb, h->qp, ko.chunk, 0.5));
+    CHK(launch(kso.outer_ds, dim3(ko.nwg), block, h->stream, h->drp.as<const int>(),
+               h->dci.as<const int>(), (const double*)h->xi.as<double>(), (const S*)h->vec_a(0),
+               h->dacc.as<double>(), h->nb, h->qp, ko.chunk, 0.5));
+  }
   HIPCHK(hipMemcpyAsync(h->dout.p, h->dacc.p, dcount * sizeof(double), hipMemcpyDeviceToDevice,
                         h->stream));
   h->alpha_M = 0;  // engine_cpu.cpp:250-252
@@ -426,8 +740,12 @@ int stoch_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
 // autodiff_matrix (engine_cpu.cpp:255-304; engine_cusparse.cpp:955-1039)
 // ---------------------------------------------------------------------------
 template <class S>
-int autodiff_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
-  const dim3 grid(h->nwg), block(fkd::WG);
+int autodiff_t(fkpm_handle* h, int M, const double* c) {
+  const dim3 block(fkd::WG);
+  const KCfg& ka = h->kc[K_AD];
+  const KCfg& ko = h->kc[K_OUT];
+  const auto& ksa = h->ks<S>(K_AD);
+  const auto& kso = h->ks<S>(K_OUT);
   const int* rp = h->rp.as<const int>();
   const int* ci = h->ci.as<const int>();
   const S* hv = h->hv.as<const S>();
@@ -440,8 +758,8 @@ int autodiff_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
   for (int m = 1; m < M / 2; m++) diag -= c[2 * m + 1];
   CHK(zero_doubles(h, dacc, dcount));
   // D = diag R R^+
-  CHK(launch(ks.outer_ss, grid, block, h->stream, drp, dci, (const S*)h->vec_a(0),
-             (const S*)h->vec_a(0), dacc, h->nb, h->qp, h->chunk, diag));
+  CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, (const S*)h->vec_a(0),
+             (const S*)h->vec_a(0), dacc, h->nb, h->qp, ko.chunk, diag));
   int launches = 0;
   CHK(timing_begin(h));
   if (M > 2) {
@@ -461,20 +779,20 @@ int autodiff_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
       const double c4 = 4 * c[2 * m], c2 = 2 * c[2 * m + 1];
       const double w = m > 1 ? 2 * c[2 * m - 1] : 0.0;
       if (h->d_fused) {
-        CHK(launch(ks.ad_fused, grid, block, h->stream, rp, ci, hv, am, am1, bm1, bm2, aout, bout,
-                   dacc, h->nb, h->qp, h->chunk, c4, c2, w, sb, sb2, 2.0));
+        CHK(launch(ksa.ad_fused, dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
+                   aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
       } else {
         // D += 2 alpha_m beta_{m+1}^+ on D's own pattern, then the step.
-        CHK(launch(ks.outer_ss, grid, block, h->stream, drp, dci, am, first ? am : bm1, dacc,
-                   h->nb, h->qp, h->chunk, first ? 2.0 * sb : 2.0));
-        CHK(launch(ks.ad_plain, grid, block, h->stream, rp, ci, hv, am, am1, bm1, bm2, aout, bout,
-                   dacc, h->nb, h->qp, h->chunk, c4, c2, w, sb, sb2, 2.0));
+        CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, am, first ? am : bm1,
+                   dacc, h->nb, h->qp, ko.chunk, first ? 2.0 * sb : 2.0));
+        CHK(launch(ksa.ad_plain, dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
+                   aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
       }
       launches++;
     }
     // D += alpha_0 beta_1^+
-    CHK(launch(ks.outer_ss, grid, block, h->stream, drp, dci, (const S*)h->A[0].p,
-               (const S*)h->Bv[1].p, dacc, h->nb, h->qp, h->chunk, 1.0));
+    CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, (const S*)h->A[0].p,
+               (const S*)h->Bv[1].p, dacc, h->nb, h->qp, ko.chunk, 1.0));
   }
   CHK(timing_end(h, &h->ms_ad, &h->n_ad, launches));
   // symmetrize + scale(1/mag)
@@ -491,7 +809,6 @@ int autodiff_t(fkpm_handle* h, const fkd::KSet<S>& ks, int M, const double* c) {
   return 0;
 }
 
-// Upload the unscaled (or scaled) H in the storage type S from the host type.
 template <class TH, class S, bool CX>
 int upload_H_t(fkpm_handle* h, DevBuf& dst, const void* val, double mag, double shift) {
   const size_t cnt = (size_t)h->nnzb * h->b * h->b;
@@ -664,7 +981,7 @@ int lanczos_t(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const
     // w = H v1, alpha = Re<w, v1>
     CHK(launch(ks.mom, grid, block, h->stream, (const int*)B.rp.p, (const int*)B.ci.p,
                (const double*)B.hv.p, (const double*)B.v1.p, (const double*)nullptr,
-               (double*)B.w.p, nb, qp, chunk, 1.0, 0.0, (double*)B.part.p));
+               (double*)B.w.p, nb, qp, chunk, 1.0, 0.0, (double*)B.part.p, 0, no_rsource<double>()));
     CHK(launch(&fkd::k_reduce_partials, dim3(1), block, h->stream, (const double*)B.part.p, nwg,
                (double*)B.ps.p));
     HIPCHK(hipMemcpyAsync(ps, B.ps.p, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -702,6 +1019,7 @@ int lanczos_t(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const
   *hi = emax + slack;
   return 0;
 }
+
 
 }  // namespace
 
@@ -813,8 +1131,8 @@ int fkpm_moments(fkpm_handle* h, int M, double* mu) {
   if (M < 2 || M % 2 != 0) return fail(FKPM_E_ARG, "moments: M=%d must be even and >= 2", M);
   CHK(ensure_ready(h));
   h->alpha_M = 0;
-  if (h->single) return moments_t<float>(h, h->kf, M, mu);
-  return moments_t<double>(h, h->kd, M, mu);
+  if (h->single) return moments_t<float>(h, M, mu);
+  return moments_t<double>(h, M, mu);
 }
 
 int fkpm_set_D_pattern(fkpm_handle* h, int nbD, int nnzbD, const int* row_ptr, const int* row_idx,
@@ -864,9 +1182,9 @@ int fkpm_stoch_matrix(fkpm_handle* h, int M, const double* c, void* D_val) {
   CHK(check_D_ready(h));
   h->d_valid = false;
   if (h->single)
-    CHK(stoch_t<float>(h, h->kf, M, c));
+    CHK(stoch_t<float>(h, M, c));
   else
-    CHK(stoch_t<double>(h, h->kd, M, c));
+    CHK(stoch_t<double>(h, M, c));
   h->d_valid = true;
   if (D_val) return copy_D_out(h, D_val);
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -886,9 +1204,9 @@ int fkpm_autodiff_matrix(fkpm_handle* h, int M, const double* c, void* D_val) {
     return fail(FKPM_E_ARG, "autodiff_matrix: a block (i,j) with i >= j of D has no (j,i) partner");
   h->d_valid = false;
   if (h->single)
-    CHK(autodiff_t<float>(h, h->kf, M, c));
+    CHK(autodiff_t<float>(h, M, c));
   else
-    CHK(autodiff_t<double>(h, h->kd, M, c));
+    CHK(autodiff_t<double>(h, M, c));
   h->d_valid = true;
   if (D_val) return copy_D_out(h, D_val);
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -940,27 +1258,99 @@ int fkpm_get_timing(fkpm_handle* h, double* ms_moments, int* n_moments, double* 
 int fkpm_get_config(fkpm_handle* h, int* lanes, int* kq, int* s_padded, int* n_wg,
                     int* fused_autodiff) {
   if (!h) return fail(FKPM_E_ARG, "null handle");
-  if (lanes) *lanes = h->L;
-  if (kq) *kq = h->KQ;
+  if (lanes) *lanes = h->kc[K_MOM].L;
+  if (kq) *kq = h->kc[K_MOM].KQ;
   if (s_padded) *s_padded = h->sp;
-  if (n_wg) *n_wg = h->nwg;
+  if (n_wg) *n_wg = h->kc[K_MOM].nwg;
   if (fused_autodiff) *fused_autodiff = h->d_fused ? 1 : 0;
+  return 0;
+}
+
+int fkpm_get_kernel_config(fkpm_handle* h, int kind, int* lanes, int* kq, int* n_wg, int* nt) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  if (kind < 0 || kind >= NKIND) return fail(FKPM_E_ARG, "bad kernel kind %d", kind);
+  const KCfg& c = h->kc[kind];
+  if (lanes) *lanes = c.L;
+  if (kq) *kq = c.KQ;
+  if (n_wg) *n_wg = c.nwg;
+  if (nt) *nt = c.nt;
+  return 0;
+}
+
+int fkpm_get_state(fkpm_handle* h, int* r_compact, int* tuned, int* moments_M) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  if (r_compact) *r_compact = h->r_compact ? 1 : 0;
+  if (tuned) *tuned = h->tuned ? 1 : 0;
+  if (moments_M) *moments_M = h->alpha_M;
   return 0;
 }
 
 int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg) {
   if (!h) return fail(FKPM_E_ARG, "null handle");
-  if (lanes != 0) {
-    fkd::GeomChoice gs[32];
-    const int ng = fkd::geometries_for_block(h->b > 0 ? h->b : 1, gs, 32);
-    bool ok = false;
-    for (int k = 0; k < ng; k++) ok |= (gs[k].L == lanes && gs[k].KQ == (kq > 0 ? kq : 1));
-    if (h->b > 0 && !ok) return fail(FKPM_E_UNSUP, "geometry lanes=%d kq=%d not compiled for block %d", lanes, kq, h->b);
-  }
+  if (lanes < 0 || kq < 0 || max_wg < 0) return fail(FKPM_E_ARG, "bad geometry");
+  if (lanes != 0 && h->b > 0 && !compiled(h->b, lanes, kq > 0 ? kq : 1))
+    return fail(FKPM_E_UNSUP, "geometry lanes=%d kq=%d not compiled for block %d", lanes, kq, h->b);
   h->force_L = lanes;
   h->force_KQ = kq;
   h->max_wg = max_wg;
   h->r_dirty = true;
+  h->alpha_M = 0;
+  return 0;
+}
+
+int fkpm_autotune(fkpm_handle* h) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  CHK(ensure_ready(h));
+  if (h->force_L > 0) return fail(FKPM_E_STATE, "a forced geometry is set (fkpm_set_geometry)");
+  CHK(tune(h));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fkpm_set_option(fkpm_handle* h, int option, int value) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  switch (option) {
+    case FKPM_OPT_STOCH_METHOD:
+      if (value != FKPM_STOCH_CLENSHAW && value != FKPM_STOCH_FORWARD)
+        return fail(FKPM_E_ARG, "bad stoch_matrix method %d", value);
+      h->stoch_method = value;
+      return 0;
+    case FKPM_OPT_AUTOTUNE:
+      h->autotune_on_use = value != 0;
+      return 0;
+  }
+  return fail(FKPM_E_ARG, "unknown option %d", option);
+}
+
+int fkpm_bandwidth_probe(fkpm_handle* h, size_t bytes, int reps, double* gbs) {
+  if (!h || !gbs || bytes == 0 || bytes % 16 != 0 || reps < 1)
+    return fail(FKPM_E_ARG, "bandwidth_probe: bad arguments");
+  HIPCHK(hipSetDevice(h->device));
+  DevBuf a, b;
+  struct Rel {
+    DevBuf *x, *y;
+    ~Rel() {
+      x->release();
+      y->release();
+    }
+  } rel{&a, &b};
+  CHK(a.ensure(bytes));
+  CHK(b.ensure(bytes));
+  const size_t n = bytes / 16;
+  CHK(zero_doubles(h, a.p, n * 2));
+  const int grid = 1024;
+  CHK(launch(&fkd::k_copy16, dim3(grid), dim3(fkd::WG), h->stream, (const double2*)a.p,
+             (double2*)b.p, n));
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < reps; r++)
+    CHK(launch(&fkd::k_copy16, dim3(grid), dim3(fkd::WG), h->stream, (const double2*)a.p,
+               (double2*)b.p, n));
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
   return 0;
 }
 

@@ -9,18 +9,21 @@
 
 namespace fkd {
 
+template <typename S>
+struct RSource;
+
 template <class S>
 struct KSet {
   void (*mom)(const int*, const int*, const S*, const S*, const S*, S*, int, int, int, double,
-              double, double*) = nullptr;
+              double, double*, int, RSource<S>) = nullptr;
   void (*xi)(const int*, const int*, const S*, const S*, const S*, S*, double*, int, int, int,
-             double, double, double, double, int) = nullptr;
+             double, double, double, double, int, int) = nullptr;
   void (*ad_fused)(const int*, const int*, const S*, const S*, const S*, const S*, const S*, S*,
                    S*, double*, int, int, int, double, double, double, double, double,
-                   double) = nullptr;
+                   double, int) = nullptr;
   void (*ad_plain)(const int*, const int*, const S*, const S*, const S*, const S*, const S*, S*,
                    S*, double*, int, int, int, double, double, double, double, double,
-                   double) = nullptr;
+                   double, int) = nullptr;
   void (*outer_ss)(const int*, const int*, const S*, const S*, double*, int, int, int,
                    double) = nullptr;
   void (*outer_sd)(const int*, const int*, const S*, const double*, double*, int, int, int,

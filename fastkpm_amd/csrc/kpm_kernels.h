@@ -74,6 +74,45 @@ __device__ __forceinline__ void stv(T* __restrict__ p, const A* v) {
   }
 }
 
+// Streaming variants: the nontemporal hint keeps data that is read or
+// written once per step from displacing the gathered vector in the caches.
+template <typename T, int NSC, typename A>
+__device__ __forceinline__ void ldv_nt(const T* __restrict__ p, A* v, bool nt) {
+  if (!nt) {
+    ldv<T, NSC>(p, v);
+    return;
+  }
+  typedef typename Vec16<T>::V VT;
+  constexpr int PER = Vec16<T>::N;
+#pragma unroll
+  for (int c = 0; c < NSC / PER; c++) {
+    VT t = __builtin_nontemporal_load(reinterpret_cast<const VT*>(p + c * PER));
+#pragma unroll
+    for (int e = 0; e < PER; e++) v[c * PER + e] = (A)t[e];
+  }
+}
+
+template <typename T, int NSC, typename A>
+__device__ __forceinline__ void stv_nt(T* __restrict__ p, const A* v, bool nt) {
+  if (!nt) {
+    stv<T, NSC>(p, v);
+    return;
+  }
+  typedef typename Vec16<T>::V VT;
+  constexpr int PER = Vec16<T>::N;
+#pragma unroll
+  for (int c = 0; c < NSC / PER; c++) {
+    VT t;
+#pragma unroll
+    for (int e = 0; e < PER; e++) t[e] = (T)v[c * PER + e];
+    __builtin_nontemporal_store(t, reinterpret_cast<VT*>(p + c * PER));
+  }
+}
+
+// Bits of the `nt` kernel argument.
+constexpr int NT_LOAD = 1;   // loads of the operands read once per step
+constexpr int NT_STORE = 2;  // stores of the step's outputs
+
 // Workgroup index remapped so that XCD x (blockIdx % 8 == x) owns a
 // contiguous range of logical workgroups.
 __device__ __forceinline__ int logical_wg() {
@@ -163,8 +202,15 @@ struct Geo {
   static constexpr int BBS = B * B * HS;         // scalars per H block
   static constexpr int DE = B * B;               // elements per D block
   static constexpr int DOWN = (DE + L - 1) / L;  // D elements owned per lane
-  // neighbour blocks gathered per batch (bounded by the registers they hold)
-  static constexpr int U = B <= 2 ? (KQ == 1 ? 4 : 2) : (B == 4 && KQ == 1 ? 2 : 1);
+  // Neighbour blocks gathered per batch for NX gathered vectors: a batch holds
+  // U * (NX*B*KQ*NS + B*B*HS) values; about 48 keep the wave's registers in
+  // check while giving several blocks of loads in flight.
+  template <int NX>
+  static constexpr int batch() {
+    constexpr int per = NX * B * KQ * NS + B * B * HS;
+    constexpr int u = 48 / per;
+    return u < 1 ? 1 : (u > 4 ? 4 : u);
+  }
 };
 
 struct RowRange {
@@ -188,9 +234,8 @@ __device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* 
                                            const S* const (&X)[NX], size_t rs, size_t qo, int p0,
                                            int p1, A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk) {
   using GE = Geo<S, CX, B, L, KQ>;
-  // two gathered vectors hold twice the registers per block: half the batch
   constexpr int NS = GE::NS, HS = GE::HS, BBS = GE::BBS;
-  constexpr int U = NX == 1 ? GE::U : (GE::U > 1 ? GE::U / 2 : 1);
+  constexpr int U = GE::template batch<NX>();
   for (int p = p0; p < p1; p += U) {
     int j[U];
 #pragma unroll
@@ -243,17 +288,54 @@ __device__ __forceinline__ void load_rows(const S* __restrict__ X, int i, size_t
       ldv<S, NS>(X + (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS, v[k][bi]);
 }
 
+// Optional source term cr * R_i of a Chebyshev step (the Clenshaw form of
+// stoch_matrix): R dense (rd, the N x Sp probe block) or compact (one nonzero
+// per orbital row: column rcol[r] (-1 for none) with value rval[r]).
+template <typename S>
+struct RSource {
+  const S* rd;
+  const int* rcol;
+  const S* rval;
+  double cr;
+};
+
+// y[e] += cr * R(row, packet q) for the NS scalars of one packet.
+template <typename S, bool CX, typename A>
+__device__ __forceinline__ void add_rsource(const RSource<S>& rs_, size_t row, size_t off, int q,
+                                            A* y) {
+  constexpr int NS = Vec16<S>::N, HS = CX ? 2 : 1, NE = NS / HS;
+  const A cr = (A)rs_.cr;
+  if (rs_.rd != nullptr) {
+    A r[NS];
+    ldv<S, NS>(rs_.rd + off, r);
+#pragma unroll
+    for (int e = 0; e < NS; e++) y[e] = fma(cr, r[e], y[e]);
+  } else {
+    const int c = rs_.rcol[row] - q * NE;  // element of this packet holding the nonzero
+    if (c >= 0 && c < NE) {
+#pragma unroll
+      for (int e = 0; e < NE; e++)
+        if (e == c) {
+#pragma unroll
+          for (int h = 0; h < HS; h++) y[e * HS + h] = fma(cr, (A)rs_.rval[row * HS + h], y[e * HS + h]);
+        }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Moments step: y = cx * (Hs x) + cz * z  (z may be null), and the two moment
-// dots sum |x_i|^2 and Re<y_i, x_i> over this workgroup's rows (i = own rows).
-// This is the forward recursion alpha_{m+1} = 2 Hs alpha_m - alpha_{m-1} of
-// Engine_CPU::moments (engine_cpu.cpp:63-84) fused with both cdot's.
+// Moments step: y = cx * (Hs x) + cz * z  (z may be null) [+ cr * R], and the
+// two moment dots sum |x_i|^2 and Re<y_i, x_i> over this workgroup's rows
+// (i = own rows; skipped when partial is null).  This is the forward
+// recursion alpha_{m+1} = 2 Hs alpha_m - alpha_{m-1} of Engine_CPU::moments
+// (engine_cpu.cpp:63-84) fused with both cdot's; with the R term it is one
+// step b_k = c_k R + 2 Hs b_{k+1} - b_{k+2} of the Clenshaw recurrence.
 // ---------------------------------------------------------------------------
 template <typename S, bool CX, int B, int L, int KQ>
 __global__ __launch_bounds__(WG) void k_step_moments(
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
-    int chunk, double cx, double cz, double* __restrict__ partial) {
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
   using A = S;
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS;
@@ -290,14 +372,16 @@ __global__ __launch_bounds__(WG) void k_step_moments(
           const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
           A zv[NS], out[NS];
           if (z != nullptr) {
-            ldv<S, NS>(z + off, zv);
+            ldv_nt<S, NS>(z + off, zv, nt & NT_LOAD);
           } else {
 #pragma unroll
             for (int e = 0; e < NS; e++) zv[e] = A(0);
           }
 #pragma unroll
           for (int e = 0; e < NS; e++) out[e] = fma(acx, acc[0][k][bi][e], acz * zv[e]);
-          stv<S, NS>(y + off, out);
+          if (rsrc.cr != 0.0)
+            add_rsource<S, CX>(rsrc, (size_t)(B * i + bi), off, q0 + k * L + l, out);
+          stv_nt<S, NS>(y + off, out, nt & NT_STORE);
 #pragma unroll
           for (int e = 0; e < NS; e++) {
             const double xe = (double)xd[k][bi][e];
@@ -308,7 +392,7 @@ __global__ __launch_bounds__(WG) void k_step_moments(
       }
     }
   }
-  wg_sum2(dmm, dm1, partial + 2 * (size_t)blockIdx.x);
+  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -321,7 +405,7 @@ __global__ __launch_bounds__(WG) void k_step_xi(
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y,
     double* __restrict__ xi, int nb, int qp, int chunk, double cx, double cz, double c0,
-    double c1, int first) {
+    double c1, int first, int nt) {
   using A = S;
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS;
@@ -355,14 +439,14 @@ __global__ __launch_bounds__(WG) void k_step_xi(
           const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
           A zv[NS], out[NS];
           if (z != nullptr) {
-            ldv<S, NS>(z + off, zv);
+            ldv_nt<S, NS>(z + off, zv, nt & NT_LOAD);
           } else {
 #pragma unroll
             for (int e = 0; e < NS; e++) zv[e] = A(0);
           }
 #pragma unroll
           for (int e = 0; e < NS; e++) out[e] = fma(acx, acc[0][k][bi][e], acz * zv[e]);
-          stv<S, NS>(y + off, out);
+          stv_nt<S, NS>(y + off, out, nt & NT_STORE);
           double xa[NS];
           if (first) {
             A xo[NS];
@@ -370,11 +454,11 @@ __global__ __launch_bounds__(WG) void k_step_xi(
 #pragma unroll
             for (int e = 0; e < NS; e++) xa[e] = fma(c0, (double)xo[e], c1 * (double)out[e]);
           } else {
-            ldv<double, NS>(xi + off, xa);
+            ldv_nt<double, NS>(xi + off, xa, nt & NT_LOAD);
 #pragma unroll
             for (int e = 0; e < NS; e++) xa[e] = fma(c1, (double)out[e], xa[e]);
           }
-          stv<double, NS>(xi + off, xa);
+          stv_nt<double, NS>(xi + off, xa, nt & NT_STORE);
         }
       }
     }
@@ -397,7 +481,7 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
     const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
     const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
     double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
-    double sb, double sb2, double dscale) {
+    double sb, double sb2, double dscale, int nt) {
   using A = S;
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS;
@@ -480,12 +564,12 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
         for (int k = 0; k < KQ; k++) {
           const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
           A a1[NS], an[NS], bo[NS], b2v[NS];
-          ldv<S, NS>(am1 + off, a1);
+          ldv_nt<S, NS>(am1 + off, a1, nt & NT_LOAD);
 #pragma unroll
           for (int e = 0; e < NS; e++) an[e] = fma(a2, acc[0][k][bi][e], -a1[e]);
-          stv<S, NS>(a_out + off, an);
+          stv_nt<S, NS>(a_out + off, an, nt & NT_STORE);
           if (bm2 != nullptr) {
-            ldv<S, NS>(bm2 + off, b2v);
+            ldv_nt<S, NS>(bm2 + off, b2v, nt & NT_LOAD);
           } else {
 #pragma unroll
             for (int e = 0; e < NS; e++) b2v[e] = asb2 * a1[e];
@@ -498,7 +582,7 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
             v = fma(aw, an[e], v);
             bo[e] = v;
           }
-          stv<S, NS>(b_out + off, bo);
+          stv_nt<S, NS>(b_out + off, bo, nt & NT_STORE);
         }
       }
     }
@@ -626,6 +710,13 @@ __global__ __launch_bounds__(WG) void k_reduce_partials(const double* __restrict
   wg_sum2(a, b, out + 2 * (size_t)blockIdx.x);
 }
 
+// Plain streaming copy of n 16-byte elements (the bandwidth probe).
+__global__ __launch_bounds__(WG) void k_copy16(const double2* __restrict__ a,
+                                               double2* __restrict__ b, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * WG + threadIdx.x; k < n; k += (size_t)gridDim.x * WG)
+    b[k] = a[k];
+}
+
 // Fill n doubles with zero (graph-capturable, unlike hipMemsetAsync).
 __global__ void k_zero(double* __restrict__ p, size_t n) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
@@ -660,6 +751,39 @@ __global__ __launch_bounds__(WG) void k_pack_R(const TH* __restrict__ src, S* __
       for (int h = 0; h < HS; h++) dst[((size_t)r * sp + c) * HS + h] = (S)tile[cc][rr * HS + h];
     }
   }
+}
+
+// Compact form of R when every row has at most one nonzero (the probing
+// colours of set_R_correlated): rcol[r] = its column (-1 for a zero row),
+// rval[r] = its value; *flag is set when some row has two or more nonzeros.
+// One thread per row; consecutive threads read consecutive rows of a column.
+template <typename TH, typename S, bool CX>
+__global__ __launch_bounds__(WG) void k_compact_R(const TH* __restrict__ src, int n, int s,
+                                                  int* __restrict__ rcol, S* __restrict__ rval,
+                                                  int* __restrict__ flag) {
+  constexpr int HS = CX ? 2 : 1;
+  const int r = blockIdx.x * WG + threadIdx.x;
+  if (r >= n) return;
+  int cnt = 0, col = -1;
+  TH v[HS];
+#pragma unroll
+  for (int h = 0; h < HS; h++) v[h] = TH(0);
+  for (int c = 0; c < s; c++) {
+    const TH* e = src + ((size_t)c * n + r) * HS;
+    bool nz = false;
+#pragma unroll
+    for (int h = 0; h < HS; h++) nz |= (e[h] != TH(0));
+    if (nz) {
+      cnt++;
+      col = c;
+#pragma unroll
+      for (int h = 0; h < HS; h++) v[h] = e[h];
+    }
+  }
+  rcol[r] = cnt == 1 ? col : -1;
+#pragma unroll
+  for (int h = 0; h < HS; h++) rval[(size_t)r * HS + h] = (S)v[h];
+  if (cnt > 1) atomicOr(flag, 1);
 }
 
 // Hs values: host type TH -> S with (H - avg I)/mag applied, as

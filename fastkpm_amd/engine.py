@@ -253,6 +253,36 @@ class HipEngine:
     def set_geometry(self, lanes: int = 0, kq: int = 0, max_wg: int = 0) -> None:
         call("fkpm_set_geometry", self._h, int(lanes), int(kq), int(max_wg))
 
+    def autotune(self) -> None:
+        """Time the kernel geometries on the current H and R and keep the fastest."""
+        call("fkpm_autotune", self._h)
+
+    def set_stoch_method(self, method: str) -> None:
+        """'clenshaw' (default) or 'forward' (the reference's summation order)."""
+        v = {"clenshaw": _capi.STOCH_CLENSHAW, "forward": _capi.STOCH_FORWARD}[method]
+        call("fkpm_set_option", self._h, _capi.OPT_STOCH_METHOD, v)
+
+    def set_autotune_on_use(self, on: bool) -> None:
+        call("fkpm_set_option", self._h, _capi.OPT_AUTOTUNE, 1 if on else 0)
+
+    def kernel_config(self, kind: int) -> dict:
+        v = [C.c_int() for _ in range(4)]
+        call("fkpm_get_kernel_config", self._h, int(kind), *[C.byref(x) for x in v])
+        return {"lanes": v[0].value, "kq": v[1].value, "n_wg": v[2].value, "nt": v[3].value}
+
+    def state(self) -> dict:
+        v = [C.c_int() for _ in range(3)]
+        call("fkpm_get_state", self._h, *[C.byref(x) for x in v])
+        return {"r_compact": bool(v[0].value), "tuned": bool(v[1].value),
+                "moments_M": v[2].value}
+
+
+def bandwidth_probe(eng: "HipEngine", nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Read+write GB/s of a plain streaming copy of nbytes on eng's device."""
+    g = C.c_double()
+    call("fkpm_bandwidth_probe", eng.handle, int(nbytes), int(reps), C.byref(g))
+    return g.value
+
 
 def device_count() -> int:
     return int(lib().fkpm_device_count())

@@ -26,6 +26,8 @@
 #ifndef FASTKPM_HIP_H
 #define FASTKPM_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -131,9 +133,46 @@ int fkpm_get_timing(fkpm_handle* h, double* ms_moments, int* n_moments, double* 
 int fkpm_get_config(fkpm_handle* h, int* lanes, int* kq, int* s_padded, int* n_wg,
                     int* fused_autodiff);
 
-/* Force a kernel geometry (lanes per block row, packets per lane; 0 = pick
- * automatically) for the next fkpm_set_R.  Used by the autotuner. */
+/* Force one kernel geometry (lanes per block row, packets per lane, workgroup
+ * cap; lanes = 0 returns to the default / tuned choice) for every kernel kind.
+ * Takes effect at the next call; discards the moments state. */
 int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
+
+/* Options (fkpm_set_option). */
+#define FKPM_OPT_STOCH_METHOD 1 /* value: FKPM_STOCH_CLENSHAW (default) or FKPM_STOCH_FORWARD */
+#define FKPM_OPT_AUTOTUNE 2     /* value 1: tune the kernel geometry at first use of a shape */
+#define FKPM_STOCH_CLENSHAW 0   /* xi by the Clenshaw recurrence: one 2-vector pass per order */
+#define FKPM_STOCH_FORWARD 1    /* xi accumulated in FP64 along the forward recursion, the
+                                   reference's summation order (engine_cpu.cpp:236-244) */
+int fkpm_set_option(fkpm_handle* h, int option, int value);
+
+/* Kernel kinds of fkpm_get_kernel_config. */
+#define FKPM_KERNEL_MOMENTS 0  /* forward step (also the Clenshaw steps of stoch_matrix) */
+#define FKPM_KERNEL_XI 1       /* forward step with FP64 xi accumulation */
+#define FKPM_KERNEL_AUTODIFF 2 /* backward step with the fused outer product */
+#define FKPM_KERNEL_OUTER 3    /* sparse outer product on D's pattern */
+
+/* Autotune the geometry (lanes per block row, packets per lane, workgroup
+ * count, nontemporal hints) of every kernel kind on the current H and R by
+ * timing the candidates on the device; the result is kept for this handle and
+ * for every later handle of the same problem shape in the process.  Discards
+ * the moments state.  No counterpart in the reference. */
+int fkpm_autotune(fkpm_handle* h);
+
+/* Geometry in use for one kernel kind. */
+int fkpm_get_kernel_config(fkpm_handle* h, int kind, int* lanes, int* kq, int* n_wg, int* nt);
+
+/* r_compact: R has at most one nonzero per row (probing colours) and the
+ * Clenshaw source term reads the compact form; tuned: the geometry came from
+ * the autotuner; moments_M: M of the held moments state (0 if none). */
+int fkpm_get_state(fkpm_handle* h, int* r_compact, int* tuned, int* moments_M);
+
+/* Bandwidth probe: copies `bytes` (a multiple of 16) between two device
+ * buffers with a plain streaming kernel (k_copy16), `reps` times, and reports
+ * the achieved read+write GB/s.  The measured ceiling the step kernels are set
+ * against, and the known-size kernel that calibrates HBM byte counters.  No
+ * counterpart in the reference. */
+int fkpm_bandwidth_probe(fkpm_handle* h, size_t bytes, int reps, double* gbs);
 
 #ifdef __cplusplus
 }

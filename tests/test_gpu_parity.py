@@ -190,3 +190,100 @@ def test_autodiff_requires_moments(oracle):
     eng.moments(8)
     with pytest.raises(FkpmError):  # different M
         eng.autodiff_matrix(np.ones(10), H.like())
+
+
+def _pair(oracle, shape=(8, 8), b=2, seed=3, uncorrelated_S=None, **kw):
+    from fastkpm_amd import lattice, EnergyScale
+    H = lattice.lattice(shape, b=b, seed=seed, **kw)
+    n = H.n_rows * b
+    if uncorrelated_S:
+        R = oracle.set_R_uncorrelated(n, uncorrelated_S, oracle.MT19937(seed))
+    else:
+        groups = lattice.colouring(shape, tuple(2 for _ in shape), b)
+        R = seeded_R(oracle, groups, seed)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    lo, hi = lam[0] - 0.2, lam[-1] + 0.3
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    return H, R, EnergyScale(lo, hi), orc
+
+
+@pytest.mark.parametrize("method", ["clenshaw", "forward"])
+@pytest.mark.parametrize("dense_R", [False, True])
+def test_stoch_matrix_methods(oracle, method, dense_R):
+    """Both stoch_matrix formulations, with compact (probing) and dense R."""
+    H, R, es, orc = _pair(oracle, J=2.0, W=1.0, uncorrelated_S=12 if dense_R else None)
+    eng = _engine()
+    eng.set_stoch_method(method)
+    eng.set_R(R)
+    eng.set_H(H, es)
+    M = 90
+    c = fermi_coeffs(oracle, M, es.lo, es.hi, kT=0.05, kind=1)
+    D = H.like()
+    eng.stoch_matrix(c, D)
+    assert eng.state()["r_compact"] is (not dense_R)
+    assert rel_err(D.val, orc.stoch_matrix(c, D)) < FP64_TOL
+    # M = 2 and M = 3 (virtual Clenshaw terms only)
+    for m in (2, 3):
+        eng.stoch_matrix(c[:m], D)
+        assert rel_err(D.val, orc.stoch_matrix(c[:m], D)) < FP64_TOL, m
+
+
+def test_every_compiled_geometry(oracle):
+    """Each compiled (lanes, packets) geometry of b = 2 gives the oracle's results."""
+    from fastkpm_amd._capi import FkpmError
+    H, R, es, orc = _pair(oracle, J=1.0)
+    M = 40
+    c = fermi_coeffs(oracle, M, es.lo, es.hi)
+    mu_o = orc.moments(M)
+    d_o = orc.autodiff_matrix(c, H)
+    s_o = orc.stoch_matrix(c, H)
+    geos = [(4, 1), (4, 2), (8, 2), (16, 2), (16, 4), (32, 4), (64, 1), (64, 4)]
+    for L, KQ in geos:
+        for maxwg in (0, 7):
+            eng = _engine()
+            eng.set_R(R)
+            eng.set_H(H, es)
+            eng.set_geometry(L, KQ, maxwg)
+            assert rel_err(eng.moments(M), mu_o) < FP64_TOL, (L, KQ)
+            D = H.like()
+            eng.autodiff_matrix(c, D)
+            assert rel_err(D.val, d_o) < FP64_TOL, (L, KQ)
+            eng.stoch_matrix(c, D)
+            assert rel_err(D.val, s_o) < FP64_TOL, (L, KQ)
+            eng.close()
+    eng = _engine()
+    with pytest.raises(FkpmError):
+        eng.set_R(R)
+        eng.set_H(H, es)
+        eng.set_geometry(8, 1, 0)  # not compiled for b = 2
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_autotuned_engine(oracle, prec):
+    H, R, es, orc = _pair(oracle, shape=(16, 16), J=2.0)
+    eng = _engine("cx_double", prec)
+    eng.set_R(R)
+    eng.set_H(H, es)
+    eng.autotune()
+    assert eng.state()["tuned"]
+    M = 120
+    c = fermi_coeffs(oracle, M, es.lo, es.hi)
+    mu = eng.moments(M)
+    D = H.like()
+    eng.autodiff_matrix(c, D)
+    S = H.like()
+    eng.stoch_matrix(c, S)
+    tol = FP64_TOL if prec == "fp64" else 2e-5
+    assert rel_err(mu, orc.moments(M)) < tol
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < (tol if prec == "fp64" else 1e-4)
+    assert rel_err(S.val, orc.stoch_matrix(c, S)) < (tol if prec == "fp64" else 1e-4)
+    # a second engine of the same shape reuses the tuned geometry
+    eng2 = _engine("cx_double", prec)
+    eng2.set_R(R)
+    eng2.set_H(H, es)
+    eng2.moments(4)
+    assert eng2.state()["tuned"]
+    for k in range(3):
+        assert eng2.kernel_config(k) == eng.kernel_config(k)
