@@ -150,6 +150,69 @@ __device__ __forceinline__ double group_sum(double v) {
 
 __device__ __forceinline__ double wave_sum(double v) { return group_sum<WAVE>(v); }
 
+// The value held by lane l ^ M (M a power of two < 64): quad_perm DPP for
+// M = 1, 2; ds_swizzle in bitmask mode (lane ((l & 0x1F) ^ M) of the 32-lane
+// half) for M = 4, 8, 16; a shuffle for M = 32.
+template <int M>
+__device__ __forceinline__ double xor_move(double v) {
+  if constexpr (M == 1) {
+    return dpp_mov<0xB1>(v);
+  } else if constexpr (M == 2) {
+    return dpp_mov<0x4E>(v);
+  } else if constexpr (M < 32) {
+    constexpr int PAT = (M << 10) | 0x1F;
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)b, PAT);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), PAT);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+  } else {
+    return __shfl_xor(v, M, WAVE);
+  }
+}
+
+// Reduce-scatter of CNT values over the lanes M, 2M, ... < L of a group: at
+// each level a lane keeps the half of its values selected by lane bit M, adds
+// the partner's copy of that half and sends the other half; once one value
+// is left the remaining levels are plain butterflies.  Afterwards lane l holds
+// in v[0 .. max(NV/L, 1)) the group sums of indices rs_base<L, NV>(l) + t.
+template <int M, int L, int CNT>
+__device__ __forceinline__ void rs_level(double* v, int l) {
+  if constexpr (M < L) {
+    if constexpr (CNT > 1) {
+      constexpr int H = CNT / 2;
+      const bool up = (l & M) != 0;
+#pragma unroll
+      for (int t = 0; t < H; t++) {
+        const double send = up ? v[t] : v[t + H];
+        const double keep = up ? v[t + H] : v[t];
+        v[t] = keep + xor_move<M>(send);
+      }
+      rs_level<2 * M, L, H>(v, l);
+    } else {
+      v[0] += xor_move<M>(v[0]);
+      rs_level<2 * M, L, 1>(v, l);
+    }
+  }
+}
+
+template <int L, int NV>
+__device__ __forceinline__ void reduce_scatter(double* v, int l) {
+  static_assert((NV & (NV - 1)) == 0, "power of two");
+  rs_level<1, L, NV>(v, l);
+}
+
+// First index owned by lane l after reduce_scatter<L, NV>.
+template <int L, int NV>
+__device__ __forceinline__ int rs_base(int l) {
+  int base = 0, cnt = NV;
+#pragma unroll
+  for (int m = 1; m < L && cnt > 1; m <<= 1) {
+    cnt >>= 1;
+    if (l & m) base += cnt;
+  }
+  return base;
+}
+
 // Sum two per-thread doubles over the workgroup; thread 0 writes them.
 __device__ __forceinline__ void wg_sum2(double a, double b, double* out) {
   __shared__ double sm[2][WG / WAVE];
@@ -255,6 +318,19 @@ __device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* 
                          xv[u][x][bj][k]);
 #pragma unroll
         for (int t = 0; t < BBS; t++) hh[u][t] = (A)hv[(size_t)(p + u) * BBS + t];
+      } else {
+        // defined zeros past the row's end keep the batch's registers from
+        // staying live across the (skipped) arithmetic
+#pragma unroll
+        for (int x = 0; x < NX; x++)
+#pragma unroll
+          for (int bj = 0; bj < B; bj++)
+#pragma unroll
+            for (int k = 0; k < KQ; k++)
+#pragma unroll
+              for (int e = 0; e < NS; e++) xv[u][x][bj][k][e] = A(0);
+#pragma unroll
+        for (int t = 0; t < BBS; t++) hh[u][t] = A(0);
       }
     }
 #pragma unroll
@@ -263,14 +339,13 @@ __device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* 
 #pragma unroll
       for (int bj = 0; bj < B; bj++)
 #pragma unroll
-        for (int bi = 0; bi < B; bi++) {
-          const A hr = hh[u][(bj * B + bi) * HS];
-          const A hi = CX ? hh[u][(bj * B + bi) * HS + HS - 1] : A(0);
+        for (int x = 0; x < NX; x++)
 #pragma unroll
-          for (int x = 0; x < NX; x++)
+          for (int k = 0; k < KQ; k++)
 #pragma unroll
-            for (int k = 0; k < KQ; k++) cmac<CX, NS>(acc[x][k][bi], hr, hi, xv[u][x][bj][k]);
-        }
+            for (int bi = 0; bi < B; bi++)
+              cmac<CX, NS>(acc[x][k][bi], hh[u][(bj * B + bi) * HS],
+                           CX ? hh[u][(bj * B + bi) * HS + HS - 1] : A(0), xv[u][x][bj][k]);
       blk(p + u, xv[u]);
     }
   }
@@ -518,8 +593,42 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
             for (int e = 0; e < NS; e++) acc[x][k][bi][e] = A(0);
       gather_row<S, A, CX, B, L, KQ, 2>(
           ci, hv, X, rs, qo, p0, p1, acc, [&](int p, auto& xv) {
-            if constexpr (FUSED_D) {
-              // D_p(bi, bj) += fD * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s])
+            constexpr int NV = GE::DE * HS;  // scalars of one D block
+            if constexpr (FUSED_D && NV <= 16) {
+              // D_p(bi, bj) += fD * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s]):
+              // per-lane partial sums of the whole block, then one reduce-scatter
+              double v[NV];
+#pragma unroll
+              for (int bj = 0; bj < B; bj++)
+#pragma unroll
+                for (int bi = 0; bi < B; bi++) {
+                  A re = A(0), im = A(0);
+#pragma unroll
+                  for (int k = 0; k < KQ; k++)
+#pragma unroll
+                    for (int e = 0; e < NS; e += HS) {
+                      const A ar = ai[k][bi][e], br = xv[1][bj][k][e];
+                      re = fma(ar, br, re);
+                      if constexpr (CX) {
+                        const A aim = ai[k][bi][e + 1], bim = xv[1][bj][k][e + 1];
+                        re = fma(aim, bim, re);
+                        im = fma(aim, br, im);
+                        im = fma(-ar, bim, im);
+                      }
+                    }
+                  const int de = bj * B + bi;  // column-major element index in the block
+                  v[de * HS] = (double)re;
+                  if constexpr (CX) v[de * HS + 1] = (double)im;
+                }
+              reduce_scatter<L, NV>(v, l);
+              constexpr int CNT = NV >= L ? NV / L : 1;
+              if (NV >= L || l < NV) {
+                const int base = rs_base<L, NV>(l);
+                double* d = dacc + (size_t)p * NV + base;
+#pragma unroll
+                for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], d[t]);
+              }
+            } else if constexpr (FUSED_D) {
               double own[GE::DOWN][HS];
 #pragma unroll
               for (int bj = 0; bj < B; bj++)
