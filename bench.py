@@ -29,6 +29,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+from fastkpm_amd import HipEngine, bandwidth_probe, device_count, host, lattice  # noqa: E402
+
 METRIC = ("N·M·S Chebyshev updates/s, FP32-mixed vs FP64 at ≤1e-6 energy err; "
           "p99 lat")
 HBM_PEAK_GBS = 8000.0
@@ -39,7 +41,6 @@ def _log(msg: str) -> None:
 
 
 def build_workload(name: str, seed: int):
-    from fastkpm_amd import lattice
     H, groups, cfg = lattice.make_config(name, seed=seed)
     return H, groups, cfg
 
@@ -65,8 +66,22 @@ def algorithmic_bytes(H, N: int, S: int, v: int, vacc: int = 16):
     }
 
 
+def measured_traffic(config: str, prec: str, kernel: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement of this workload
+    (profiles/traffic_<config>_<prec>.json), or None."""
+    path = ROOT / "profiles" / f"traffic_{config.lower()}_{prec}.json"
+    if not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    for k in data.get("kernels", []):
+        if kernel.split(" ")[0] in k["kernel"]:
+            return {"traffic_bytes_per_launch": k["traffic_bytes_per_launch"],
+                    "source": f"{path.name} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"}
+    return None
+
+
 def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
-    from fastkpm_amd import HipEngine, host
     M = cfg["M"]
     eng = HipEngine("cx_double", prec)
     es = eng.energy_scale(H, 0.01)
@@ -131,7 +146,6 @@ def cpu_baseline(H, groups, cfg, es, seed: int, m_sample: int):
     on the same H, R and coefficients with M' = m_sample moments."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_py
-    from fastkpm_amd import host
     R = host.set_R_correlated("cx_double", groups, host.RNG(seed))
     S = R.shape[1]
     N = H.n_rows * H.b_len
@@ -174,11 +188,16 @@ def main(argv=None) -> int:
               file=sys.stderr)
         return 2
 
-    from fastkpm_amd import device_count
     if device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
 
+    # the practical streaming ceiling on this device: a plain 1 GiB copy
+    # (k_copy16); its dispatches also calibrate the HBM byte counters of a
+    # profiled run of this script
+    probe = HipEngine("cx_double", "fp64")
+    copy_gbs = bandwidth_probe(probe, 1 << 30, 5)
+    probe.close()
     _log(f"building workload {args.config}")
     H, groups, cfg = build_workload(args.config, args.seed)
     _log(f"N={H.n_rows * H.b_len} nnzb={H.n_blocks()} M={cfg['M']}; {args.precision} run")
@@ -209,6 +228,12 @@ def main(argv=None) -> int:
                        f"wg={kc['n_wg']}, nt={kc['nt']})"),
             "bytes_per_launch": kern[dom]["bytes_per_launch"],
             "avg_launch_us": kern[dom]["avg_launch_us"]}
+    roof["copy_GBs"] = round(copy_gbs, 1)
+    roof["frac_of_copy"] = round(kern[dom]["achieved_GBs"] / copy_gbs, 4)
+    tr = measured_traffic(args.config, args.precision, names[dom])
+    if tr is not None:
+        roof["traffic"] = tr["traffic_bytes_per_launch"]
+        roof["traffic_source"] = tr["source"]
     times_ms = np.array(r["times"]) * 1e3
     out = {
         "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,

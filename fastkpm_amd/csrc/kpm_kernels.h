@@ -113,15 +113,6 @@ __device__ __forceinline__ void stv_nt(T* __restrict__ p, const A* v, bool nt) {
 constexpr int NT_LOAD = 1;   // loads of the operands read once per step
 constexpr int NT_STORE = 2;  // stores of the step's outputs
 
-// Workgroup index remapped so that XCD x (blockIdx % 8 == x) owns a
-// contiguous range of logical workgroups.
-__device__ __forceinline__ int logical_wg() {
-  const int k = blockIdx.x, n = gridDim.x;
-  const int x = k & 7, r = k >> 3;
-  const int per = n >> 3, extra = n & 7;
-  return x * per + (x < extra ? x : extra) + r;
-}
-
 // DPP lane move inside 16-lane rows (gfx9 encodings: quad_perm 0x00-0xFF,
 // row_mirror 0x140, row_half_mirror 0x141).
 template <int CTRL>
@@ -276,16 +267,29 @@ struct Geo {
   }
 };
 
+// Rows of one workgroup.  Workgroups reach the 8 XCDs round-robin
+// (blockIdx % 8); XCD x owns one contiguous region of block rows, the union
+// of the chunks of its workgroups, and its workgroups sweep that region
+// together: workgroup r of the XCD takes passes r, r + n_x, r + 2 n_x, ... of
+// `step` = (block rows per workgroup pass) rows.  The region the XCD touches at
+// a time is then about n_x * step rows wide, so the neighbour rows a gather
+// needs (a lattice's next rows) are still in that XCD's L2.
 struct RowRange {
-  int r0, r1;
+  int r0, r1, stride;
 };
 
-__device__ __forceinline__ RowRange wg_rows(int nb, int chunk) {
-  const int lw = logical_wg();
-  RowRange r;
-  r.r0 = lw * chunk;
-  r.r1 = min(nb, r.r0 + chunk);
-  return r;
+__device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step) {
+  const int k = blockIdx.x, n = gridDim.x;
+  const int x = k & 7, r = k >> 3;
+  const int per = n >> 3, extra = n & 7;
+  const int nx = per + (x < extra ? 1 : 0);        // workgroups of XCD x
+  const int first = x * per + (x < extra ? x : extra);  // its first chunk
+  RowRange rr;
+  const int start = first * chunk;
+  rr.r1 = min(nb, (first + nx) * chunk);
+  rr.r0 = start + r * step;
+  rr.stride = nx * step;
+  return rr;
 }
 
 // Block row i's gather: acc[x][k][bi] += sum_p Hs_p(bi, bj) X_x[B j_p + bj]
@@ -417,13 +421,13 @@ __global__ __launch_bounds__(WG) void k_step_moments(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
   const size_t rs = (size_t)qp * NS;  // scalars per orbital row
   const A acx = (A)cx, acz = (A)cz;
   double dmm = 0.0, dm1 = 0.0;
   const S* const X[1] = {x};
 
-  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     if (i >= rr.r1) continue;
@@ -487,12 +491,12 @@ __global__ __launch_bounds__(WG) void k_step_xi(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
   const size_t rs = (size_t)qp * NS;
   const A acx = (A)cx, acz = (A)cz;
   const S* const X[1] = {x};
 
-  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     if (i >= rr.r1) continue;
@@ -564,7 +568,7 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
   const size_t rs = (size_t)qp * NS;
   const bool vb1 = (bm1 == nullptr);
   const double hb1 = vb1 ? sb : 1.0;  // scale of the beta_{m+1} gather
@@ -573,7 +577,7 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
   const A a2 = (A)2, ab2 = (A)(2.0 * hb1), ac4 = (A)c4, ac2 = (A)c2, aw = (A)w, asb2 = (A)sb2;
   const double fD = dscale * hb1;
 
-  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     if (i >= rr.r1) continue;
@@ -716,10 +720,10 @@ __global__ __launch_bounds__(WG) void k_outer(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
   const size_t rs = (size_t)qp * NS;  // scalars per row (same count for every type)
 
-  for (int base = rr.r0; base < rr.r1; base += GE::GWG) {
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     const int i = base + g;
     const bool active = i < rr.r1;
     const int p0 = active ? drp[i] : 0;
