@@ -13,7 +13,7 @@ LIB_PATH = _PKG / "libfastkpm_hip.so"
 
 F32, F64, C32, C64 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
-OPT_STOCH_METHOD, OPT_AUTOTUNE = 1, 2
+OPT_STOCH_METHOD, OPT_AUTOTUNE, OPT_MFMA = 1, 2, 3
 STOCH_CLENSHAW, STOCH_FORWARD = 0, 1
 KERNEL_MOMENTS, KERNEL_XI, KERNEL_AUTODIFF, KERNEL_OUTER = 0, 1, 2, 3
 
@@ -46,6 +46,7 @@ SIGNATURES = {
     "fkpm_set_option": (_I, [_P, _I, _I]),
     "fkpm_autotune": (_I, [_P]),
     "fkpm_get_kernel_config": (_I, [_P, _I, _IP, _IP, _IP, _IP]),
+    "fkpm_get_kernel_mfma": (_I, [_P, _I, _IP]),
     "fkpm_get_state": (_I, [_P, _IP, _IP, _IP]),
 }
 

@@ -24,7 +24,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["kpm_capi.hip", "kpm_inst_dc.hip", "kpm_inst_dr.hip", "kpm_inst_fc.hip",
                "kpm_inst_fr.hip"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-             # packed FP32 instructions stay off (toolchain notes)
+             # packed FP32 instructions (v_pk_*_f32) stay off: every kernel is built
+             # from plain scalar FP32 code
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", str(INCLUDE), "-I", str(CSRC)]
 

@@ -146,6 +146,7 @@ enum KKind { K_MOM = 0, K_XI = 1, K_AD = 2, K_OUT = 3, NKIND = 4 };
 // workgroup cap and nontemporal bits; nwg/chunk derived from them.
 struct KCfg {
   int L = 0, KQ = 0, maxwg = 0, nt = 0;
+  int mfma = 0;  // matrix-core kernel (one wave per block row: L = 64, KQ = 1)
   int nwg = 0, chunk = 0;
 };
 
@@ -189,6 +190,7 @@ struct fkpm_handle {
   int force_L = 0, force_KQ = 0, max_wg = 0;
   bool tuned = false;
   bool autotune_on_use = false;
+  bool mfma_on = true;  // matrix-core kernels where the block shape has them
   int stoch_method = FKPM_STOCH_CLENSHAW;
   DevBuf R, A[3], Bv[3], xi, partial, psum, tune_dacc;
   int alpha_M = 0;  // M of the moments whose final state is held, 0 if none
@@ -263,12 +265,39 @@ void derive_grid(int nb, KCfg& c) {
   c.chunk = chunk;
 }
 
+// Matrix-core kernels exist for complex 8x8 blocks (forward/Clenshaw steps
+// and the autodiff step).
+bool mfma_avail(const fkpm_handle* h, int kind) {
+  return h->mfma_on && h->cx && h->b == 8 && (kind == K_MOM || kind == K_AD);
+}
+
 int set_kind(fkpm_handle* h, int kind, KCfg c) {
+  if (c.mfma) {
+    if (!mfma_avail(h, kind))
+      return fail(FKPM_E_UNSUP, "no matrix-core kernel for block %d (kind %d)", h->b, kind);
+    c.L = 64;  // one wave per block row
+    c.KQ = 1;
+    c.nt = 0;
+  }
   if (!lookup_kset(h, kind, h->b, c.L, c.KQ))
     return fail(FKPM_E_UNSUP, "no kernel for block %d, lanes %d, packets %d", h->b, c.L, c.KQ);
   derive_grid(h->nb, c);
   h->kc[kind] = c;
   return 0;
+}
+
+// Kernel entry of a kind under its configuration.
+template <class S>
+auto mom_fn(const fkd::KSet<S>& ks, const KCfg& c) {
+  return c.mfma ? ks.mfma_mom : ks.mom;
+}
+template <class S>
+auto adf_fn(const fkd::KSet<S>& ks, const KCfg& c) {
+  return c.mfma ? ks.mfma_ad_fused : ks.ad_fused;
+}
+template <class S>
+auto adp_fn(const fkd::KSet<S>& ks, const KCfg& c) {
+  return c.mfma ? ks.mfma_ad_plain : ks.ad_plain;
 }
 
 // Default geometry of a kernel kind when nothing is tuned: one pass of width
@@ -299,6 +328,7 @@ KCfg default_cfg(fkpm_handle* h, int kind) {
   }
   c.maxwg = 4096;
   c.nt = 0;
+  if (mfma_avail(h, kind)) c.mfma = 1;
   return c;
 }
 
@@ -308,7 +338,8 @@ std::map<std::string, std::array<KCfg, NKIND>> g_tune_cache;
 
 std::string tune_key(const fkpm_handle* h) {
   char k[128];
-  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb, h->nnzb);
+  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb, h->nnzb,
+           h->mfma_on ? 1 : 0);
   return k;
 }
 
@@ -327,6 +358,8 @@ int choose_geometry(fkpm_handle* h) {
     if (w >= qp0 && (wmax == 0 || w < wmax)) wmax = w;
   }
   if (wmax == 0) wmax = widest;
+  // matrix-core tiles are 16 complex columns wide
+  if (h->mfma_on && h->cx && h->b == 8) wmax = std::max(wmax, 16 / ne);
   if (h->force_L > 0) {
     const int fkq = h->force_KQ > 0 ? h->force_KQ : 1;
     if (!compiled(h->b, h->force_L, fkq))
@@ -490,7 +523,7 @@ int tune_t(fkpm_handle* h) {
     const auto& ks = h->ks<S>(K_MOM);
     S* outs[] = {A0, A1, A2, B0, B1, B2};
     for (S* o : outs)
-      CHK(launch(ks.mom, dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R, (const S*)nullptr,
+      CHK(launch(mom_fn(ks, c), dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R, (const S*)nullptr,
                  o, h->nb, h->qp, c.chunk, 1.0, 0.0, (double*)nullptr, 0, no_rsource<S>()));
     CHK(zero_doubles(h, h->tune_dacc.p, (size_t)h->nnzb * h->b * h->b * HS));
   }
@@ -506,6 +539,7 @@ int tune_t(fkpm_handle* h) {
   for (int kind : kinds) {
     KCfg best = h->kc[kind];
     float best_ms = 1e30f;
+    std::vector<KCfg> cands;
     for (int g = 0; g < ng; g++) {
       const int w = gs[g].L * gs[g].KQ;
       if (w > h->wmax || 2 * w < h->wmax) continue;
@@ -516,6 +550,18 @@ int tune_t(fkpm_handle* h) {
           c.KQ = gs[g].KQ;
           c.maxwg = cap;
           c.nt = nt;
+          cands.push_back(c);
+        }
+    }
+    if (mfma_avail(h, kind))
+      for (int cap : caps) {
+        KCfg c;
+        c.mfma = 1;
+        c.maxwg = cap;
+        cands.push_back(c);
+      }
+    {
+      for (const KCfg& c : cands) {
           CHK(set_kind(h, kind, c));
           const KCfg& k = h->kc[kind];
           const auto& ks = h->ks<S>(kind);
@@ -528,7 +574,7 @@ int tune_t(fkpm_handle* h) {
             S* y = ring[(t + 2) % 3];
             // recurrences keep |x| bounded: y = 2 Hs x - z stays within the spectrum scale
             if (kind == K_MOM)
-              return launch(ks.mom, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
                             (const S*)x, (const S*)z, y, h->nb, h->qp, k.chunk, 2.0, -1.0,
                             h->partial.as<double>(), k.nt, no_rsource<S>());
             if (kind == K_XI)
@@ -537,7 +583,7 @@ int tune_t(fkpm_handle* h) {
                             0.0, 0.5, t == 0 ? 1 : 0, k.nt);
             // backward step: reads alpha_m, alpha_{m+1}, beta_{m+1}, beta_{m+2};
             // writes alpha_{m-1}, beta_m into the third buffers
-            return launch(ks.ad_fused, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+            return launch(adf_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
                           (const S*)x, (const S*)z, (const S*)bring[(t + 1) % 3],
                           (const S*)bring[t % 3], y, bring[(t + 2) % 3], h->tune_dacc.as<double>(),
                           h->nb, h->qp, k.chunk, 0.1, 0.1, 0.1, 1.0, 0.0, 2.0, k.nt);
@@ -550,8 +596,8 @@ int tune_t(fkpm_handle* h) {
           float ms = 0.f;
           HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
           if (log)
-            fprintf(stderr, "[fkpm tune] kind=%d L=%d KQ=%d maxwg=%d nwg=%d nt=%d: %.2f us\n", kind,
-                    k.L, k.KQ, k.maxwg, k.nwg, k.nt, 1e3 * ms / 6);
+            fprintf(stderr, "[fkpm tune] kind=%d L=%d KQ=%d mfma=%d maxwg=%d nwg=%d nt=%d: %.2f us\n",
+                    kind, k.L, k.KQ, k.mfma, k.maxwg, k.nwg, k.nt, 1e3 * ms / 6);
           if (ms < best_ms) {
             best_ms = ms;
             best = k;
@@ -560,7 +606,9 @@ int tune_t(fkpm_handle* h) {
     }
     CHK(set_kind(h, kind, best));
   }
-  KCfg outc = h->kc[K_AD];  // outer products: once per call, the autodiff geometry
+  // outer products: once per call, the autodiff geometry (a vector-unit
+  // geometry of full pass width when autodiff runs on the matrix cores)
+  KCfg outc = h->kc[K_AD].mfma ? default_cfg(h, K_OUT) : h->kc[K_AD];
   outc.nt = 0;
   CHK(set_kind(h, K_OUT, outc));
   h->alpha_M = 0;
@@ -590,7 +638,7 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
     const S* x = (const S*)h->vec_a(m);
     const S* z = m == 0 ? nullptr : (const S*)h->vec_a(m - 1);
     S* y = (S*)h->vec_a(m + 1);
-    CHK(launch(ks.mom, grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
+    CHK(launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
                h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk, m == 0 ? 1.0 : 2.0,
                m == 0 ? 0.0 : -1.0, h->partial.as<double>() + (size_t)m * kc.nwg * 2,
                kc.nt, no_rsource<S>()));
@@ -664,7 +712,7 @@ int stoch_clenshaw(fkpm_handle* h, int M, const double* c, const S** xi_out) {
   Vb b1{R, c[M - 1]}, b2{nullptr, 0.0};
   int slot = 0, launches = 0;
   auto step = [&](double xfac, double cr, S* y) -> int {
-    return launch(ks.mom, grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
+    return launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
                   h->hv.as<const S>(), b1.p, b2.p, y, h->nb, h->qp, kc.chunk, xfac * b1.s,
                   b2.p ? -b2.s : 0.0, (double*)nullptr, kc.nt, rsource<S>(h, cr));
   };
@@ -779,13 +827,13 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
       const double c4 = 4 * c[2 * m], c2 = 2 * c[2 * m + 1];
       const double w = m > 1 ? 2 * c[2 * m - 1] : 0.0;
       if (h->d_fused) {
-        CHK(launch(ksa.ad_fused, dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
+        CHK(launch(adf_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
                    aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
       } else {
         // D += 2 alpha_m beta_{m+1}^+ on D's own pattern, then the step.
         CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, am, first ? am : bm1,
                    dacc, h->nb, h->qp, ko.chunk, first ? 2.0 * sb : 2.0));
-        CHK(launch(ksa.ad_plain, dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
+        CHK(launch(adp_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
                    aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
       }
       launches++;
@@ -1277,6 +1325,13 @@ int fkpm_get_kernel_config(fkpm_handle* h, int kind, int* lanes, int* kq, int* n
   return 0;
 }
 
+int fkpm_get_kernel_mfma(fkpm_handle* h, int kind, int* mfma) {
+  if (!h || !mfma) return fail(FKPM_E_ARG, "null argument");
+  if (kind < 0 || kind >= NKIND) return fail(FKPM_E_ARG, "bad kernel kind %d", kind);
+  *mfma = h->kc[kind].mfma;
+  return 0;
+}
+
 int fkpm_get_state(fkpm_handle* h, int* r_compact, int* tuned, int* moments_M) {
   if (!h) return fail(FKPM_E_ARG, "null handle");
   if (r_compact) *r_compact = h->r_compact ? 1 : 0;
@@ -1318,6 +1373,15 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
       return 0;
     case FKPM_OPT_AUTOTUNE:
       h->autotune_on_use = value != 0;
+      return 0;
+    case FKPM_OPT_MFMA:
+      if (value != 0 && value != 1) return fail(FKPM_E_ARG, "bad MFMA switch %d", value);
+      if (h->mfma_on != (value == 1)) {
+        h->mfma_on = value == 1;
+        h->r_dirty = true;  // geometry (and padded width) chosen again at next use
+        h->alpha_M = 0;
+        h->tuned = false;
+      }
       return 0;
   }
   return fail(FKPM_E_ARG, "unknown option %d", option);

@@ -30,6 +30,16 @@ struct KSet {
                    double) = nullptr;
   void (*outer_ds)(const int*, const int*, const double*, const S*, double*, int, int, int,
                    double) = nullptr;
+  // matrix-core kernels (complex 8x8 blocks only; null otherwise), same
+  // signatures as mom / ad_fused / ad_plain
+  void (*mfma_mom)(const int*, const int*, const S*, const S*, const S*, S*, int, int, int, double,
+                   double, double*, int, RSource<S>) = nullptr;
+  void (*mfma_ad_fused)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
+                        S*, S*, double*, int, int, int, double, double, double, double, double,
+                        double, int) = nullptr;
+  void (*mfma_ad_plain)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
+                        S*, S*, double*, int, int, int, double, double, double, double, double,
+                        double, int) = nullptr;
   int B = 0, L = 0, KQ = 0;
 };
 

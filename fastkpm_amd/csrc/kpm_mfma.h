@@ -1,0 +1,458 @@
+// kpm_mfma.h -- matrix-core (MFMA) step kernels for complex 8x8 BSR blocks.
+//
+// With 8 orbitals per block the block SpMM Y_i = sum_j Hs_ij X_j is a dense
+// contraction of 8x8 complex blocks against 8 x S complex column strips, at
+// about 17 flop per HBM byte in FP32: vector FMAs would make it ALU-bound, so
+// the contraction runs on the CDNA4 matrix cores in its real 16x16 form
+//   [Y_re]   [H_re  -H_im] [X_re]
+//   [Y_im] = [H_im   H_re] [X_im]
+// with v_mfma_{f32,f64}_16x16x4: four MFMAs (K = 16) per neighbour block and
+// 16-column tile.  One wave owns one block row; lane (q, c) = (lane / 16,
+// lane % 16) holds orbitals q and q + 4 of column c of a tile, both as an
+// MFMA B operand (X_j) and as accumulator rows (Y_i), so the own-row operands
+// of the axpby and of the moment dots sit in the same lane as the result.
+// The rows of the 16x16 operator are permuted to make that so (mfma_lane).
+//
+// The autodiff step's outer product D_ij += f alpha_i beta_j^+ on H's pattern
+// is the 16x16 real product G = P Q^T over the S columns (P, Q = [re; im] of
+// the two 8 x S strips), also on the matrix cores; D follows from
+//   D_re = G[a][b] + G[8+a][8+b],  D_im = G[8+a][b] - G[a][8+b].
+// G accumulates per tile in the storage type and across tiles in FP64.
+//
+// MFMA 16x16x4 lane layout (64 lanes, l = lane, register i = 0..3):
+//   A (16 x 4): lane l holds A[l % 16][l / 16]
+//   B (4 x 16): lane l holds B[l / 16][l % 16]
+//   C (16 x 16), f32: lane l holds C[4 (l / 16) + i][l % 16]
+//                f64: lane l holds C[l / 16 + 4 i][l % 16]
+// (checked on gfx950).  The operator's row permutation differs between the
+// two so that in both, register i of lane (q, c) is (re, im, re, im)[i] of
+// orbital (q, q, q + 4, q + 4)[i] in column c.
+#pragma once
+
+#include "kpm_kernels.h"
+
+namespace fkd {
+
+template <typename S>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float2 c2;
+  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  typedef double2 c2;
+  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+typedef double dv4 __attribute__((ext_vector_type(4)));
+
+// Lane-constant part of the operator row this lane feeds as MFMA A operand.
+struct MfmaLane {
+  int q, c;    // lane / 16, lane % 16
+  int h0, h1;  // scalar offsets in an H block of Hs(O, q) and Hs(O, q + 4)
+  bool im;     // this A row is an imaginary-part row
+};
+
+template <typename S>
+__device__ __forceinline__ MfmaLane mfma_lane() {
+  MfmaLane m;
+  const int lane = threadIdx.x & (WAVE - 1);
+  m.q = lane >> 4;
+  m.c = lane & 15;
+  // A row r = c feeds accumulator row r: register r % 4 of lane group r / 4
+  // (f32) or register r / 4 of lane group r % 4 (f64)
+  const int grp = sizeof(S) == 4 ? (m.c >> 2) : (m.c & 3);
+  const int reg = sizeof(S) == 4 ? (m.c & 3) : (m.c >> 2);
+  const int O = grp + 4 * (reg >> 1);
+  m.im = (reg & 1) != 0;
+  m.h0 = (m.q * 8 + O) * 2;  // blocks are column-major: (col * 8 + row) * 2
+  m.h1 = ((m.q + 4) * 8 + O) * 2;
+  return m;
+}
+
+// The four A-operand values (K chunks 0..3: re/im of orbital q, re/im of
+// orbital q + 4) of one Hs block for this lane.
+template <typename S>
+__device__ __forceinline__ void mfma_a(const S* __restrict__ blk, const MfmaLane& m, S (&a)[4]) {
+  const S hr0 = blk[m.h0], hi0 = blk[m.h0 + 1];
+  const S hr1 = blk[m.h1], hi1 = blk[m.h1 + 1];
+  a[0] = m.im ? hi0 : hr0;
+  a[1] = m.im ? hr0 : -hi0;
+  a[2] = m.im ? hi1 : hr1;
+  a[3] = m.im ? hr1 : -hi1;
+}
+
+template <typename S>
+__device__ __forceinline__ typename Mfma<S>::v4 mfma_block(const S (&a)[4],
+                                                          const typename Mfma<S>::c2& x0,
+                                                          const typename Mfma<S>::c2& x1,
+                                                          typename Mfma<S>::v4 acc) {
+  using MF = Mfma<S>;
+  acc = MF::mac(a[0], x0.x, acc);
+  acc = MF::mac(a[1], x0.y, acc);
+  acc = MF::mac(a[2], x1.x, acc);
+  acc = MF::mac(a[3], x1.y, acc);
+  return acc;
+}
+
+template <typename S>
+__device__ __forceinline__ typename Mfma<S>::c2 ldc(const S* __restrict__ p) {
+  return *reinterpret_cast<const typename Mfma<S>::c2*>(p);
+}
+template <typename S>
+__device__ __forceinline__ void stc(S* __restrict__ p, typename Mfma<S>::c2 v) {
+  *reinterpret_cast<typename Mfma<S>::c2*>(p) = v;
+}
+
+// Neighbour blocks gathered per batch and 16-column tiles per pass.
+constexpr int MF_U = 4;
+constexpr int MF_TU = 2;
+
+// Y tiles t0, t0 + 1 of block row i: acc[tt] += sum_p Hs_p X_{j_p} over the
+// row's blocks [p0, p1), X gathered at column offset (scalars) co[tt].
+template <typename S>
+__device__ __forceinline__ void mfma_gather(const int* __restrict__ ci, const S* __restrict__ hv,
+                                            const S* __restrict__ X, size_t rs, const MfmaLane& m,
+                                            int p0, int p1, const size_t (&co)[MF_TU],
+                                            const bool (&tv)[MF_TU],
+                                            typename Mfma<S>::v4 (&acc)[MF_TU]) {
+  using C2 = typename Mfma<S>::c2;
+  for (int p = p0; p < p1; p += MF_U) {
+    S a[MF_U][4];
+    C2 x0[MF_U][MF_TU], x1[MF_U][MF_TU];
+#pragma unroll
+    for (int u = 0; u < MF_U; u++) {
+      if (p + u < p1) {
+        const int j = ci[p + u];
+        mfma_a<S>(hv + (size_t)(p + u) * 128, m, a[u]);
+        const size_t r0 = (size_t)(8 * j + m.q) * rs, r1 = (size_t)(8 * j + 4 + m.q) * rs;
+#pragma unroll
+        for (int tt = 0; tt < MF_TU; tt++) {
+          if (tv[tt]) {
+            x0[u][tt] = ldc<S>(X + r0 + co[tt]);
+            x1[u][tt] = ldc<S>(X + r1 + co[tt]);
+          } else {
+            x0[u][tt] = C2{S(0), S(0)};
+            x1[u][tt] = C2{S(0), S(0)};
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[u][k] = S(0);
+#pragma unroll
+        for (int tt = 0; tt < MF_TU; tt++) {
+          x0[u][tt] = C2{S(0), S(0)};
+          x1[u][tt] = C2{S(0), S(0)};
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < MF_U; u++) {
+      if (p + u >= p1) continue;
+#pragma unroll
+      for (int tt = 0; tt < MF_TU; tt++) acc[tt] = mfma_block<S>(a[u], x0[u][tt], x1[u][tt], acc[tt]);
+    }
+  }
+}
+
+// Clenshaw source term cr * R for one lane's element (row, col) pair.
+template <typename S>
+__device__ __forceinline__ void mfma_rsource(const RSource<S>& rs_, size_t row, int col,
+                                             size_t off, typename Mfma<S>::c2& y) {
+  const S cr = (S)rs_.cr;
+  if (rs_.rd != nullptr) {
+    const typename Mfma<S>::c2 r = ldc<S>(rs_.rd + off);
+    y.x = fma(cr, r.x, y.x);
+    y.y = fma(cr, r.y, y.y);
+  } else if (rs_.rcol[row] == col) {
+    y.x = fma(cr, rs_.rval[2 * row], y.x);
+    y.y = fma(cr, rs_.rval[2 * row + 1], y.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Moments / Clenshaw step on the matrix cores (complex 8x8 blocks):
+//   y = cx (Hs x) + cz z [+ cr R], dots sum |x_i|^2 and Re<y_i, x_i>.
+// Same contract as k_step_moments (engine_cpu.cpp:63-84 fused with both cdot's).
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(WG) void k_mfma_step(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
+  using C2 = typename Mfma<S>::c2;
+  using V4 = typename Mfma<S>::v4;
+  constexpr int NS = Vec16<S>::N;
+  (void)nt;
+  const MfmaLane m = mfma_lane<S>();
+  const int g = threadIdx.x / WAVE;
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE);
+  const size_t rs = (size_t)qp * NS;  // scalars per orbital row
+  const int ntile = (int)(rs / 32);   // 16 complex columns per tile
+  const S acx = (S)cx, acz = (S)cz;
+  double dmm = 0.0, dm1 = 0.0;
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    const int i = __builtin_amdgcn_readfirstlane(base + g);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    const size_t o0 = (size_t)(8 * i + m.q) * rs, o1 = (size_t)(8 * i + 4 + m.q) * rs;
+    for (int t0 = 0; t0 < ntile; t0 += MF_TU) {
+      size_t co[MF_TU];
+      bool tv[MF_TU];
+      V4 acc[MF_TU];
+#pragma unroll
+      for (int tt = 0; tt < MF_TU; tt++) {
+        co[tt] = (size_t)(t0 + tt) * 32 + 2 * m.c;
+        tv[tt] = (t0 + tt) < ntile;
+        acc[tt] = V4{S(0), S(0), S(0), S(0)};
+      }
+      mfma_gather<S>(ci, hv, x, rs, m, p0, p1, co, tv, acc);
+#pragma unroll
+      for (int tt = 0; tt < MF_TU; tt++) {
+        if (!tv[tt]) continue;
+        const size_t f0 = o0 + co[tt], f1 = o1 + co[tt];
+        const C2 x0 = ldc<S>(x + f0), x1 = ldc<S>(x + f1);
+        C2 z0{S(0), S(0)}, z1{S(0), S(0)};
+        if (z != nullptr) {
+          z0 = ldc<S>(z + f0);
+          z1 = ldc<S>(z + f1);
+        }
+        C2 y0, y1;
+        y0.x = fma(acx, acc[tt].x, acz * z0.x);
+        y0.y = fma(acx, acc[tt].y, acz * z0.y);
+        y1.x = fma(acx, acc[tt].z, acz * z1.x);
+        y1.y = fma(acx, acc[tt].w, acz * z1.y);
+        if (rsrc.cr != 0.0) {
+          const int col = (t0 + tt) * 16 + m.c;
+          mfma_rsource<S>(rsrc, (size_t)(8 * i + m.q), col, f0, y0);
+          mfma_rsource<S>(rsrc, (size_t)(8 * i + 4 + m.q), col, f1, y1);
+        }
+        stc<S>(y + f0, y0);
+        stc<S>(y + f1, y1);
+        const double a0 = x0.x, a1 = x0.y, a2 = x1.x, a3 = x1.y;
+        dmm = fma(a0, a0, fma(a1, a1, fma(a2, a2, fma(a3, a3, dmm))));
+        dm1 = fma((double)y0.x, a0,
+                  fma((double)y0.y, a1, fma((double)y1.x, a2, fma((double)y1.y, a3, dm1))));
+      }
+    }
+  }
+  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)blockIdx.x);
+}
+
+// Neighbour blocks of one pass of the autodiff kernel: the FP64 G
+// accumulators of the fused outer product are held per block of the pass.
+constexpr int MF_DMAX = 8;
+
+// ---------------------------------------------------------------------------
+// Autodiff backward step on the matrix cores (complex 8x8 blocks); same
+// contract as k_step_autodiff (engine_cpu.cpp:278-294):
+//   a_out = 2 Hs alpha_m - alpha_{m+1}
+//   b_out = 4 c_{2m} alpha_m + 2 c_{2m+1} alpha_{m+1} + 2 Hs beta_{m+1}
+//           - beta_{m+2} + w alpha_{m-1}
+//   D    += dscale alpha_m beta_{m+1}^+ on H's pattern (FUSED_D)
+// bm1 == null: beta_{m+1} = sb alpha_m; bm2 == null: beta_{m+2} = sb2 alpha_{m+1}.
+// A block row with more than MF_DMAX blocks takes several passes over its
+// tiles; between passes the partial products Hs alpha_m, Hs beta_{m+1} are
+// parked in a_out / b_out, which only this wave touches.
+// ---------------------------------------------------------------------------
+template <typename S, bool FUSED_D>
+__global__ __launch_bounds__(WG) void k_mfma_autodiff(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  using C2 = typename Mfma<S>::c2;
+  using V4 = typename Mfma<S>::v4;
+  using MF = Mfma<S>;
+  constexpr int NS = Vec16<S>::N;
+  (void)nt;
+  const MfmaLane m = mfma_lane<S>();
+  const int g = threadIdx.x / WAVE;
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE);
+  const size_t rs = (size_t)qp * NS;
+  const int ntile = (int)(rs / 32);
+  const bool vb1 = (bm1 == nullptr);
+  const double hb1 = vb1 ? sb : 1.0;
+  const S* __restrict__ xb = vb1 ? am : bm1;  // gathered beta_{m+1} (or alpha_m, scaled by sb)
+  const S a2 = (S)2, ab2 = (S)(2.0 * hb1), ac4 = (S)c4, ac2 = (S)c2, aw = (S)w, asb2 = (S)sb2;
+  const double fD = dscale * hb1;
+  // D-operand layout: lane l holds row l % 16 (orbital c & 7, part c >> 3)
+  // of column 4 kappa + q of a tile
+  const int dorb = m.c & 7, dpart = m.c >> 3;
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    const int i = __builtin_amdgcn_readfirstlane(base + g);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    const size_t o0 = (size_t)(8 * i + m.q) * rs, o1 = (size_t)(8 * i + 4 + m.q) * rs;
+    const size_t od = (size_t)(8 * i + dorb) * rs + dpart;
+    for (int pc = p0; pc < p1 || pc == p0; pc += MF_DMAX) {
+      const int pe = min(pc + MF_DMAX, p1);
+      const bool first = (pc == p0), last = (pe == p1);
+      dv4 G[MF_DMAX];
+#pragma unroll
+      for (int u = 0; u < MF_DMAX; u++) G[u] = dv4{0.0, 0.0, 0.0, 0.0};
+      for (int t = 0; t < ntile; t++) {
+        const size_t co = (size_t)t * 32 + 2 * m.c;
+        const size_t f0 = o0 + co, f1 = o1 + co;
+        V4 accA = V4{S(0), S(0), S(0), S(0)}, accB = accA;
+        if (!first) {  // partial products parked by the previous pass
+          const C2 pa0 = ldc<S>(a_out + f0), pa1 = ldc<S>(a_out + f1);
+          const C2 pb0 = ldc<S>(b_out + f0), pb1 = ldc<S>(b_out + f1);
+          accA = V4{pa0.x, pa0.y, pa1.x, pa1.y};
+          accB = V4{pb0.x, pb0.y, pb1.x, pb1.y};
+        }
+        S pv[4];
+        if constexpr (FUSED_D) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) pv[k] = am[od + 2 * ((size_t)t * 16 + 4 * k + m.q)];
+        }
+#pragma unroll
+        for (int u0 = 0; u0 < MF_DMAX; u0 += 2) {
+          if (pc + u0 >= pe) break;
+          S a[2][4];
+          C2 xa0[2], xa1[2], xb0[2], xb1[2];
+          S qv[2][4];
+#pragma unroll
+          for (int v = 0; v < 2; v++) {
+            const int p = pc + u0 + v;
+            if (p < pe) {
+              const int j = ci[p];
+              mfma_a<S>(hv + (size_t)p * 128, m, a[v]);
+              const size_t r0 = (size_t)(8 * j + m.q) * rs + co, r1 = (size_t)(8 * j + 4 + m.q) * rs + co;
+              xa0[v] = ldc<S>(am + r0);
+              xa1[v] = ldc<S>(am + r1);
+              xb0[v] = ldc<S>(xb + r0);
+              xb1[v] = ldc<S>(xb + r1);
+              if constexpr (FUSED_D) {
+                const size_t jd = (size_t)(8 * j + dorb) * rs + dpart;
+#pragma unroll
+                for (int k = 0; k < 4; k++) qv[v][k] = xb[jd + 2 * ((size_t)t * 16 + 4 * k + m.q)];
+              }
+            } else {
+#pragma unroll
+              for (int k = 0; k < 4; k++) {
+                a[v][k] = S(0);
+                qv[v][k] = S(0);
+              }
+              xa0[v] = xa1[v] = xb0[v] = xb1[v] = C2{S(0), S(0)};
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < 2; v++) {
+            if (pc + u0 + v >= pe) continue;
+            accA = mfma_block<S>(a[v], xa0[v], xa1[v], accA);
+            accB = mfma_block<S>(a[v], xb0[v], xb1[v], accB);
+            if constexpr (FUSED_D) {
+              V4 gt = V4{S(0), S(0), S(0), S(0)};
+#pragma unroll
+              for (int k = 0; k < 4; k++) gt = MF::mac(pv[k], qv[v][k], gt);
+              G[u0 + v] += dv4{(double)gt.x, (double)gt.y, (double)gt.z, (double)gt.w};
+            }
+          }
+        }
+        if (!last) {
+          stc<S>(a_out + f0, C2{accA.x, accA.y});
+          stc<S>(a_out + f1, C2{accA.z, accA.w});
+          stc<S>(b_out + f0, C2{accB.x, accB.y});
+          stc<S>(b_out + f1, C2{accB.z, accB.w});
+          continue;
+        }
+        const C2 ai0 = ldc<S>(am + f0), ai1 = ldc<S>(am + f1);
+        const C2 a10 = ldc<S>(am1 + f0), a11 = ldc<S>(am1 + f1);
+        C2 b20, b21;
+        if (bm2 != nullptr) {
+          b20 = ldc<S>(bm2 + f0);
+          b21 = ldc<S>(bm2 + f1);
+        } else {
+          b20 = C2{asb2 * a10.x, asb2 * a10.y};
+          b21 = C2{asb2 * a11.x, asb2 * a11.y};
+        }
+        const S hx[4] = {accA.x, accA.y, accA.z, accA.w};
+        const S hb[4] = {accB.x, accB.y, accB.z, accB.w};
+        const S ai[4] = {ai0.x, ai0.y, ai1.x, ai1.y};
+        const S a1[4] = {a10.x, a10.y, a11.x, a11.y};
+        const S b2[4] = {b20.x, b20.y, b21.x, b21.y};
+        S an[4], bo[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          an[e] = fma(a2, hx[e], -a1[e]);
+          S v = fma(ab2, hb[e], -b2[e]);
+          v = fma(ac4, ai[e], v);
+          v = fma(ac2, a1[e], v);
+          v = fma(aw, an[e], v);
+          bo[e] = v;
+        }
+        stc<S>(a_out + f0, C2{an[0], an[1]});
+        stc<S>(a_out + f1, C2{an[2], an[3]});
+        stc<S>(b_out + f0, C2{bo[0], bo[1]});
+        stc<S>(b_out + f1, C2{bo[2], bo[3]});
+      }
+      if constexpr (FUSED_D && sizeof(S) == 4) {
+        // f32 layout: lane (q, c) holds G[4q + i][c]; lanes q < 2, c < 8 own
+        // D(a = 4q + i, b = c)
+        const bool owner = (m.q < 2) && (m.c < 8);
+#pragma unroll
+        for (int u = 0; u < MF_DMAX; u++) {
+          if (pc + u >= pe) break;
+          double dre[4], dim[4];
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const double gv = G[u][e];
+            const double g40 = __shfl_xor(gv, 40);  // G[8 + a][8 + b]
+            const double g32 = __shfl_xor(gv, 32);  // G[8 + a][b]
+            const double g08 = __shfl_xor(gv, 8);   // G[a][8 + b]
+            dre[e] = gv + g40;
+            dim[e] = g32 - g08;
+          }
+          if (owner) {
+            double* d = dacc + ((size_t)(pc + u) * 64 + m.c * 8 + 4 * m.q) * 2;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              d[2 * e] = fma(fD, dre[e], d[2 * e]);
+              d[2 * e + 1] = fma(fD, dim[e], d[2 * e + 1]);
+            }
+          }
+        }
+      } else if constexpr (FUSED_D) {
+        // f64 layout: lane (q, c) holds G[q + 4i][c]: G[a][c] (i = 0, 1) and
+        // G[8 + a][c] (i = 2, 3) for a = q + 4i; lanes c < 8 own D(a, b = c)
+        const bool owner = m.c < 8;
+#pragma unroll
+        for (int u = 0; u < MF_DMAX; u++) {
+          if (pc + u >= pe) break;
+          double dre[2], dim[2];
+#pragma unroll
+          for (int e = 0; e < 2; e++) {
+            const double gr = G[u][e], gi = G[u][e + 2];
+            const double xr = __shfl_xor(gr, 8);  // G[a][8 + b]
+            const double xi = __shfl_xor(gi, 8);  // G[8 + a][8 + b]
+            dre[e] = gr + xi;
+            dim[e] = gi - xr;
+          }
+          if (owner) {
+            double* d = dacc + ((size_t)(pc + u) * 64 + m.c * 8 + m.q) * 2;
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+              d[8 * e] = fma(fD, dre[e], d[8 * e]);
+              d[8 * e + 1] = fma(fD, dim[e], d[8 * e + 1]);
+            }
+          }
+        }
+      }
+      if (p1 == p0) break;
+    }
+  }
+}
+
+}  // namespace fkd

@@ -265,10 +265,18 @@ class HipEngine:
     def set_autotune_on_use(self, on: bool) -> None:
         call("fkpm_set_option", self._h, _capi.OPT_AUTOTUNE, 1 if on else 0)
 
+    def set_mfma(self, on: bool) -> None:
+        """Matrix-core kernels where the block shape has them (complex 8x8
+        blocks; on by default), or the vector-unit kernels only."""
+        call("fkpm_set_option", self._h, _capi.OPT_MFMA, 1 if on else 0)
+
     def kernel_config(self, kind: int) -> dict:
         v = [C.c_int() for _ in range(4)]
         call("fkpm_get_kernel_config", self._h, int(kind), *[C.byref(x) for x in v])
-        return {"lanes": v[0].value, "kq": v[1].value, "n_wg": v[2].value, "nt": v[3].value}
+        mf = C.c_int()
+        call("fkpm_get_kernel_mfma", self._h, int(kind), C.byref(mf))
+        return {"lanes": v[0].value, "kq": v[1].value, "n_wg": v[2].value, "nt": v[3].value,
+                "mfma": bool(mf.value)}
 
     def state(self) -> dict:
         v = [C.c_int() for _ in range(3)]

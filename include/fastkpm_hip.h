@@ -141,6 +141,8 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 /* Options (fkpm_set_option). */
 #define FKPM_OPT_STOCH_METHOD 1 /* value: FKPM_STOCH_CLENSHAW (default) or FKPM_STOCH_FORWARD */
 #define FKPM_OPT_AUTOTUNE 2     /* value 1: tune the kernel geometry at first use of a shape */
+#define FKPM_OPT_MFMA 3         /* value 1 (default): matrix-core (MFMA) kernels where the block
+                                   shape has them (complex 8x8 blocks); 0: vector-unit kernels */
 #define FKPM_STOCH_CLENSHAW 0   /* xi by the Clenshaw recurrence: one 2-vector pass per order */
 #define FKPM_STOCH_FORWARD 1    /* xi accumulated in FP64 along the forward recursion, the
                                    reference's summation order (engine_cpu.cpp:236-244) */
@@ -161,6 +163,10 @@ int fkpm_autotune(fkpm_handle* h);
 
 /* Geometry in use for one kernel kind. */
 int fkpm_get_kernel_config(fkpm_handle* h, int kind, int* lanes, int* kq, int* n_wg, int* nt);
+
+/* *mfma = 1 if that kernel kind runs on the matrix cores.  No counterpart in
+ * the reference. */
+int fkpm_get_kernel_mfma(fkpm_handle* h, int kind, int* mfma);
 
 /* r_compact: R has at most one nonzero per row (probing colours) and the
  * Clenshaw source term reads the compact form; tuned: the geometry came from

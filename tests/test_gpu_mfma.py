@@ -1,0 +1,177 @@
+"""Matrix-core (MFMA) kernels for complex 8x8 blocks, through the C-ABI,
+against the CPU oracle on identical seeded inputs.  FP64: 1e-10 relative
+(norm-wise, max|a-b|/max|b|); FP32-mixed: energy 1e-6, moments 2e-5 and D
+1e-4 relative."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import fermi_coeffs, rel_err, seeded_R
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+K_MOM, K_XI, K_AD, K_OUT = 0, 1, 2, 3
+
+
+def _dense_rows_b8(nb=24, extra=6, seed=3):
+    """Hermitian 8x8-block matrix whose rows hold up to ~2*extra+1 blocks (more
+    than one pass of the autodiff kernel's per-row block budget)."""
+    from fastkpm_amd import lattice
+    rng = np.random.default_rng(seed)
+    rows, cols, blocks = [], [], []
+    for i in range(nb):
+        x = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+        rows.append(i)
+        cols.append(i)
+        blocks.append((x + x.conj().T) / 2)
+        for j in rng.choice([k for k in range(nb) if k != i], size=extra, replace=False):
+            t = 0.5 * (rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8)))
+            rows += [i, int(j)]
+            cols += [int(j), i]
+            blocks += [t, t.conj().T]
+    return lattice.coo_to_bsr(nb, 8, rows, cols, np.array(blocks))
+
+
+def _lattice_b8(shape=(4, 4, 2), seed=7):
+    from fastkpm_amd import lattice
+    return lattice.lattice(shape, b=8, orbital_blocks=True, seed=seed)
+
+
+def _setup(oracle, H, R, prec="fp64", mfma=True):
+    from fastkpm_amd import HipEngine, EnergyScale
+    lam = np.linalg.eigvalsh(H.to_dense())
+    lo, hi = lam[0] - 0.1, lam[-1] + 0.1
+    eng = HipEngine("cx_double", prec)
+    eng.set_mfma(mfma)
+    eng.set_R(R)
+    eng.set_H(H, EnergyScale(lo, hi))
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    return eng, orc, lo, hi
+
+
+def _probing_R(oracle, H, seed=11):
+    groups = (np.arange(8)[None, :] + 8 * (np.arange(H.n_rows) % 3)[:, None]).reshape(-1)
+    return seeded_R(oracle, groups.astype(np.int32), seed)
+
+
+def _check_all(oracle, eng, orc, H, lo, hi, M, tol_mu, tol_d):
+    c = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
+    assert rel_err(eng.moments(M), orc.moments(M)) < tol_mu
+    D = H.like()
+    eng.autodiff_matrix(c, D)
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < tol_d
+    cf = fermi_coeffs(oracle, M, lo, hi, kT=0.05, kind=1)
+    eng.stoch_matrix(cf, D)
+    assert rel_err(D.val, orc.stoch_matrix(cf, D)) < tol_d
+
+
+@pytest.mark.parametrize("case", ["lattice", "dense_rows"])
+def test_mfma_fp64_matches_oracle(oracle, case):
+    H = _lattice_b8() if case == "lattice" else _dense_rows_b8()
+    assert case != "dense_rows" or int(np.max(np.diff(H.row_ptr))) > 8
+    R = _probing_R(oracle, H)
+    eng, orc, lo, hi = _setup(oracle, H, R)
+    eng.moments(2)
+    assert eng.kernel_config(K_MOM)["mfma"] and eng.kernel_config(K_AD)["mfma"]
+    _check_all(oracle, eng, orc, H, lo, hi, 64, FP64_TOL, FP64_TOL)
+
+
+def test_mfma_equals_vector_kernels(oracle):
+    """The matrix-core and vector-unit paths agree with each other and the oracle."""
+    H = _dense_rows_b8(nb=16, extra=5, seed=9)
+    R = _probing_R(oracle, H, seed=5)
+    out = {}
+    for mf in (True, False):
+        eng, orc, lo, hi = _setup(oracle, H, R, mfma=mf)
+        M = 40
+        c = fermi_coeffs(oracle, M, lo, hi)
+        mu = eng.moments(M)
+        assert eng.kernel_config(K_MOM)["mfma"] is mf
+        D = H.like()
+        eng.autodiff_matrix(c, D)
+        out[mf] = (mu, D.val.copy())
+    assert rel_err(out[True][0], out[False][0]) < 1e-12
+    assert rel_err(out[True][1], out[False][1]) < 1e-12
+
+
+@pytest.mark.parametrize("S_cols", [1, 3, 17, 40])
+def test_mfma_ragged_columns(oracle, S_cols):
+    """Dense R with a column count that leaves padded columns in the last tile."""
+    H = _lattice_b8(shape=(3, 3, 2), seed=2)
+    n = H.n_rows * H.b_len
+    R = oracle.set_R_uncorrelated(n, S_cols, oracle.MT19937(4 + S_cols))
+    eng, orc, lo, hi = _setup(oracle, H, R)
+    assert eng.config()["s_padded"] % 16 == 0
+    _check_all(oracle, eng, orc, H, lo, hi, 36, FP64_TOL, FP64_TOL)
+
+
+@pytest.mark.parametrize("method", ["clenshaw", "forward"])
+def test_mfma_stoch_methods_dense_R(oracle, method):
+    H = _dense_rows_b8(nb=12, extra=4, seed=21)
+    n = H.n_rows * H.b_len
+    R = oracle.set_R_uncorrelated(n, 20, oracle.MT19937(8))
+    eng, orc, lo, hi = _setup(oracle, H, R)
+    eng.set_stoch_method(method)
+    M = 50
+    c = fermi_coeffs(oracle, M, lo, hi, kT=0.05, kind=1)
+    D = H.like()
+    eng.stoch_matrix(c, D)
+    assert rel_err(D.val, orc.stoch_matrix(c, D)) < FP64_TOL
+    for m in (2, 3):
+        eng.stoch_matrix(c[:m], D)
+        assert rel_err(D.val, orc.stoch_matrix(c[:m], D)) < FP64_TOL, m
+
+
+def test_mfma_foreign_D_pattern(oracle):
+    """D on the on-site blocks only: the unfused matrix-core autodiff step plus
+    the sparse outer product."""
+    from fastkpm_amd import lattice
+    H = _lattice_b8(shape=(3, 3, 2), seed=12)
+    nb = H.n_rows
+    D = lattice.coo_to_bsr(nb, 8, list(range(nb)), list(range(nb)), np.zeros((nb, 8, 8)))
+    R = _probing_R(oracle, H, seed=3)
+    eng, orc, lo, hi = _setup(oracle, H, R)
+    M = 44
+    eng.moments(M)
+    orc.moments(M)
+    c = fermi_coeffs(oracle, M, lo, hi)
+    eng.autodiff_matrix(c, D)
+    assert eng.config()["fused_autodiff"] is False
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < FP64_TOL
+
+
+def test_mfma_fp32_mixed(oracle):
+    """FP32 matrix-core recursion with FP64 dots and D against the FP64 oracle."""
+    H = _lattice_b8(shape=(4, 4, 2), seed=31)
+    R = _probing_R(oracle, H, seed=6)
+    eng, orc, lo, hi = _setup(oracle, H, R, prec="fp32")
+    M = 80
+    c = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
+    mu = eng.moments(M)
+    mu_o = orc.moments(M)
+    assert eng.kernel_config(K_MOM)["mfma"]
+    assert rel_err(mu, mu_o) < 2e-5
+    e, e_o = float(np.dot(c, mu)), float(np.dot(c, mu_o))
+    assert abs(e - e_o) / abs(e_o) < 1e-6
+    D = H.like()
+    eng.autodiff_matrix(c, D)
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < 1e-4
+    eng.stoch_matrix(c, D)
+    assert rel_err(D.val, orc.stoch_matrix(c, D)) < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_mfma_autotuned(oracle, prec):
+    H = _lattice_b8(shape=(4, 4, 4), seed=5)
+    R = _probing_R(oracle, H, seed=2)
+    eng, orc, lo, hi = _setup(oracle, H, R, prec=prec)
+    eng.autotune()
+    assert eng.state()["tuned"]
+    if prec == "fp64":
+        _check_all(oracle, eng, orc, H, lo, hi, 32, FP64_TOL, FP64_TOL)
+    else:
+        _check_all(oracle, eng, orc, H, lo, hi, 32, 2e-5, 1e-4)
