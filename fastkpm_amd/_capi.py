@@ -48,7 +48,21 @@ SIGNATURES = {
     "fkpm_get_kernel_config": (_I, [_P, _I, _IP, _IP, _IP, _IP]),
     "fkpm_get_kernel_mfma": (_I, [_P, _I, _IP]),
     "fkpm_get_state": (_I, [_P, _IP, _IP, _IP]),
+    # batched engine
+    "fkpm_batch_footprint": (_I, [_I, _I, _I, _I, _I, _I, C.POINTER(C.c_size_t)]),
+    "fkpm_batch_open": (_I, [_I, _I, C.c_size_t, _I, C.POINTER(_P)]),
+    "fkpm_batch_close": (_I, [_P]),
+    "fkpm_batch_set_option": (_I, [_P, _I, _I]),
+    "fkpm_batch_submit": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _D, _I, _I, _P, _I, _DP,
+                               C.POINTER(C.c_longlong)]),
+    "fkpm_batch_submit_probing": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _D, _I, _I, _IP, _P,
+                                       _I, _DP, C.POINTER(C.c_longlong)]),
+    "fkpm_batch_poll": (_I, [_P, C.c_longlong, _IP]),
+    "fkpm_batch_wait": (_I, [_P, C.c_longlong, _DP, _DP, _DP, _DP]),
+    "fkpm_batch_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                              C.POINTER(C.c_longlong), _IP, C.POINTER(C.c_size_t)]),
 }
+BATCH_OPT_GRAPH, BATCH_OPT_TICKS = 1, 2
 
 _lib = None
 

@@ -1,10 +1,20 @@
 // C entry points of include/fastkpm_host.h over the C++ host code.
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <vector>
 
 #include "fastkpm.h"
 #include "fastkpm_hip.h"
 #include "fastkpm_host.h"
+
+namespace fkpm {
+// probing phases (defined with the Engine<T> base class, engine.cpp)
+template <typename T>
+T random_phase(RNG& rng);
+template <>
+cx_double random_phase<cx_double>(RNG& rng);
+}  // namespace fkpm
 
 namespace {
 
@@ -136,6 +146,42 @@ int fkpm_host_set_R_correlated(int dtype, int n, const int* groups, void* rng, i
   int rc = 0;
   int g = guarded([&] { rc = by_dtype<CorF>(dtype, n, groups, rng, j0, j1, R_out, s_out); });
   return g ? g : rc;
+}
+
+int fkpm_host_set_R_probing(int n, const int* groups, void* rng, int* rcol_out, void* rval_out,
+                            int* s_out) {
+  return guarded([&] {
+    if (n <= 0 || !groups || !rng || !rcol_out || !rval_out)
+      throw std::invalid_argument("set_R_probing: bad arguments");
+    const auto mm = std::minmax_element(groups, groups + n);
+    if (*mm.first != 0) throw std::invalid_argument("set_R_correlated: colours must start at 0");
+    const int s = *mm.second + 1;
+    fkpm::cx_double* rv = static_cast<fkpm::cx_double*>(rval_out);
+    if (s >= n) {  // identity (Engine::set_R_correlated, src/engine.cpp:103-116)
+      for (int i = 0; i < n; i++) {
+        rcol_out[i] = i;
+        rv[i] = fkpm::cx_double(1.0, 0.0);
+      }
+      if (s_out) *s_out = n;
+      return;
+    }
+    // column j draws its phases from RNG(rng()) in row order over its rows
+    // (src/engine.cpp:76-89), rows bucketed by colour once
+    std::vector<int> cnt(s + 1, 0);
+    for (int i = 0; i < n; i++) cnt[groups[i] + 1]++;
+    for (int k = 1; k <= s; k++) cnt[k] += cnt[k - 1];
+    std::vector<int> rows(n), pos(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n; i++) rows[pos[groups[i]]++] = i;
+    fkpm::RNG& master = *static_cast<fkpm::RNG*>(rng);
+    for (int j = 0; j < s; j++) {
+      fkpm::RNG rng_j(master());
+      for (int k = cnt[j]; k < cnt[j + 1]; k++) {
+        rcol_out[rows[k]] = j;
+        rv[rows[k]] = fkpm::random_phase<fkpm::cx_double>(rng_j);
+      }
+    }
+    if (s_out) *s_out = s;
+  });
 }
 
 int fkpm_host_expansion_coefficients(int M, int Mq, int kind, double kT, double mu, double lo,

@@ -1,0 +1,1028 @@
+// kpm_batch.hip -- batched multi-Hamiltonian engine: the serving runtime.
+//
+// Many KPM requests (each: a Hamiltonian, probing vectors R, M Chebyshev
+// moments and coefficients c) share the GPU.  A request admitted to a slot
+// runs, entirely on the device, the pipeline of one Engine solve:
+//   moments(M)                     ticks 0 .. M/2-1    (forward steps)
+//   D = diag R R^+                 tick  M/2
+//   autodiff backward steps        ticks M/2+1 .. M-1
+//   D += alpha_0 beta_1^+          tick  M            (M > 2)
+//   symmetrize + scale(1/mag),
+//   reduction of the moment dots   tick  M+1
+// (Engine_CPU::moments / autodiff_matrix, engine_cpu.cpp:63-84, 255-304).
+// One tick advances every active slot by one step: per class of slots
+// (precision, block size, column pass width) one forward, one backward and
+// one auxiliary launch cover all slots of the class, each workgroup decoding
+// its slot from a prefix table; slots in another phase exit at once.  A run
+// of ticks is captured once in a hipGraph and replayed while the slot set is
+// unchanged.  Between replays a scheduler thread retires finished requests
+// (moments and D to the host) and admits queued ones while their HBM
+// footprint fits the arena and a slot is free (continuous batching).
+//
+// Each slot's kernels see only that slot's buffers with a workgroup count
+// fixed by the request's shape, so a request's results are bitwise the same
+// alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "fastkpm_hip.h"
+#include "kpm_internal.h"
+#include "kpm_kernels.h"
+#include "kpm_mfma.h"
+
+namespace fkd {
+
+// Device descriptor of one admitted request.
+struct SlotDev {
+  const int* rp;
+  const int* ci;
+  const void* hv;
+  const int* tpos;
+  const void* R;
+  void* A[3];
+  void* Bv[3];
+  double* dacc;
+  double* dout;
+  double* partial;  // [M/2][nwg][2] per-workgroup moment dots
+  double* psum;     // [M/2][2]
+  const double* c;  // M coefficients
+  int nb, qp, chunk, nwg, M, pad;
+  double inv_mag, diag, sb;
+};
+
+// Slot of workgroup `bid`: the last s with wg_off[s] <= bid.
+__device__ __forceinline__ int batch_find(const int* __restrict__ wg_off, int ns, int bid) {
+  int lo = 0, hi = ns;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (wg_off[mid] <= bid) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <typename S>
+__device__ __forceinline__ S* vec_a(const SlotDev& d, int k) {
+  return k == 0 ? (S*)d.R : (S*)d.A[k % 3];
+}
+
+// Forward Chebyshev step of every slot of the class in its forward phase.
+template <typename S, int B, int L, int KQ, bool MF>
+__global__ __launch_bounds__(WG) void k_batch_fwd(const SlotDev* __restrict__ sd,
+                                                  const int* __restrict__ st,
+                                                  const int* __restrict__ slots,
+                                                  const int* __restrict__ wg_off, int ns) {
+  const int s = batch_find(wg_off, ns, (int)blockIdx.x);
+  const int slot = slots[s];
+  const SlotDev& d = sd[slot];
+  const int step = st[slot];
+  const int h = d.M / 2;
+  if (step >= h) return;
+  const WgId wid{(int)blockIdx.x - wg_off[s], d.nwg};
+  const S* x = vec_a<S>(d, step);
+  const S* z = step == 0 ? nullptr : vec_a<S>(d, step - 1);
+  S* y = vec_a<S>(d, step + 1);
+  const double cx = step == 0 ? 1.0 : 2.0, cz = step == 0 ? 0.0 : -1.0;
+  double* part = d.partial + (size_t)step * d.nwg * 2;
+  RSource<S> nors{nullptr, nullptr, nullptr, 0.0};
+  if constexpr (MF)
+    mfma_step_dev<S>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp, d.chunk, cx, cz, part,
+                     0, nors);
+  else
+    step_moments_dev<S, true, B, L, KQ>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp,
+                                        d.chunk, cx, cz, part, 0, nors);
+}
+
+// Backward (autodiff) step of every slot of the class in its backward phase;
+// the sequence of autodiff_t in the single-request engine.
+template <typename S, int B, int L, int KQ, bool MF>
+__global__ __launch_bounds__(WG) void k_batch_bwd(const SlotDev* __restrict__ sd,
+                                                  const int* __restrict__ st,
+                                                  const int* __restrict__ slots,
+                                                  const int* __restrict__ wg_off, int ns) {
+  const int s = batch_find(wg_off, ns, (int)blockIdx.x);
+  const int slot = slots[s];
+  const SlotDev& d = sd[slot];
+  const int step = st[slot];
+  const int h = d.M / 2;
+  if (step <= h || step >= d.M) return;
+  const int m = h - 1 - (step - h - 1);  // m = M/2-1 .. 1
+  const WgId wid{(int)blockIdx.x - wg_off[s], d.nwg};
+  const S* am = (const S*)d.A[m % 3];
+  const S* am1 = (const S*)d.A[(m + 1) % 3];
+  S* aout = (S*)d.A[(m - 1) % 3];
+  const bool first = (m == h - 1), second = (m == h - 2);
+  const S* bm1 = first ? nullptr : (const S*)d.Bv[(m + 1) % 3];
+  const S* bm2 = (first || second) ? nullptr : (const S*)d.Bv[(m + 2) % 3];
+  const double sb2 = second ? d.sb : 0.0;
+  S* bout = (S*)d.Bv[m % 3];
+  const double c4 = 4 * d.c[2 * m], c2 = 2 * d.c[2 * m + 1];
+  const double w = m > 1 ? 2 * d.c[2 * m - 1] : 0.0;
+  if constexpr (MF)
+    mfma_autodiff_dev<S, true>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2, aout, bout,
+                               d.dacc, d.nb, d.qp, d.chunk, c4, c2, w, d.sb, sb2, 2.0, 0);
+  else
+    step_autodiff_dev<S, true, B, L, KQ, true>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2,
+                                               aout, bout, d.dacc, d.nb, d.qp, d.chunk, c4, c2, w,
+                                               d.sb, sb2, 2.0, 0);
+}
+
+// Auxiliary work of a tick: zero D (tick 0), D = diag R R^+ (tick M/2),
+// D += alpha_0 beta_1^+ (tick M), symmetrize/scale D and reduce the moment
+// dots (tick M+1).
+template <typename S, int B, int L, int KQ, bool MF>
+__global__ __launch_bounds__(WG) void k_batch_aux(const SlotDev* __restrict__ sd,
+                                                  const int* __restrict__ st,
+                                                  const int* __restrict__ slots,
+                                                  const int* __restrict__ wg_off, int ns) {
+  const int s = batch_find(wg_off, ns, (int)blockIdx.x);
+  const int slot = slots[s];
+  const SlotDev& d = sd[slot];
+  const int step = st[slot];
+  const int h = d.M / 2;
+  const WgId wid{(int)blockIdx.x - wg_off[s], d.nwg};
+  const int* rp = d.rp;
+  const size_t nnzb = (size_t)rp[d.nb];
+  const size_t delem = nnzb * B * B;  // complex elements of D
+  const size_t t0 = (size_t)wid.k * WG + threadIdx.x, tstride = (size_t)wid.n * WG;
+  if (step == 0) {
+    for (size_t e = t0; e < 2 * delem; e += tstride) d.dacc[e] = 0.0;
+  } else if (step == h) {
+    outer_dev<S, S, S, true, B, L, KQ>(wid, rp, d.ci, (const S*)d.R, (const S*)d.R, d.dacc, d.nb,
+                                       d.qp, d.chunk, d.diag);
+  } else if (step == d.M && d.M > 2) {
+    outer_dev<S, S, S, true, B, L, KQ>(wid, rp, d.ci, (const S*)d.A[0], (const S*)d.Bv[1], d.dacc,
+                                       d.nb, d.qp, d.chunk, 1.0);
+  } else if (step == d.M + 1) {
+    // D_out = ((D + D^+) / 2) / mag on the block pattern (symmetrize + scale)
+    for (size_t e = t0; e < delem; e += tstride) {
+      const size_t k = e / (B * B);
+      const int r = (int)(e % (B * B));
+      const int bj = r / B, bi = r % B;
+      const int tp = d.tpos[k];
+      if (tp < 0) {
+        d.dout[2 * e] = d.dacc[2 * e] * d.inv_mag;
+        d.dout[2 * e + 1] = d.dacc[2 * e + 1] * d.inv_mag;
+      } else {
+        const size_t e2 = (size_t)tp * B * B + (size_t)bi * B + bj;
+        d.dout[2 * e] = (d.dacc[2 * e] + d.dacc[2 * e2]) / 2.0 * d.inv_mag;
+        d.dout[2 * e + 1] = (d.dacc[2 * e + 1] - d.dacc[2 * e2 + 1]) / 2.0 * d.inv_mag;
+      }
+    }
+    // moment dots of forward step m: fixed-order sum over the nwg partials
+    for (int m = wid.k; m < h; m += wid.n) {
+      const double* p = d.partial + (size_t)m * d.nwg * 2;
+      double a = 0.0, b = 0.0;
+      for (int w = threadIdx.x; w < d.nwg; w += WG) {
+        a += p[2 * w];
+        b += p[2 * w + 1];
+      }
+      wg_sum2(a, b, d.psum + 2 * (size_t)m);
+    }
+  }
+}
+
+// Probing vectors from their compact form (one nonzero per orbital row:
+// column rcol[i] (-1 for none), value rval[i]) into the packed n x sp block.
+template <typename S>
+__global__ __launch_bounds__(WG) void k_batch_scatter_R(const int* __restrict__ rcol,
+                                                        const double* __restrict__ rval, int n,
+                                                        int sp, S* __restrict__ R) {
+  const size_t total = (size_t)n * sp;
+  for (size_t e = (size_t)blockIdx.x * WG + threadIdx.x; e < total; e += (size_t)gridDim.x * WG) {
+    const int i = (int)(e / sp), j = (int)(e % sp);
+    const bool hit = rcol[i] == j;
+    R[2 * e] = hit ? (S)rval[2 * (size_t)i] : S(0);
+    R[2 * e + 1] = hit ? (S)rval[2 * (size_t)i + 1] : S(0);
+  }
+}
+
+// End of a tick: every active slot moves to its next step.
+__global__ void k_batch_advance(int* __restrict__ st, const int* __restrict__ act, int na) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < na) st[act[t]] += 1;
+}
+
+}  // namespace fkd
+
+// ---------------------------------------------------------------------------
+// Host runtime
+// ---------------------------------------------------------------------------
+namespace {
+
+using Clock = std::chrono::steady_clock;
+using fkd::SlotDev;
+typedef void (*BatchKernel)(const SlotDev*, const int*, const int*, const int*, int);
+
+// Kernel classes: (storage type, block size, lanes, packets per lane, MFMA).
+// W = pass width in 16-byte packets (the padding granularity of S).
+struct ClassDef {
+  int B, L, KQ, mf, W;
+  BatchKernel fwd, bwd, aux;
+};
+
+template <typename S, int B, int L, int KQ, bool MF>
+ClassDef mk_class(int W) {
+  return ClassDef{B, L, KQ, MF ? 1 : 0, W, &fkd::k_batch_fwd<S, B, L, KQ, MF>,
+                  &fkd::k_batch_bwd<S, B, L, KQ, MF>, &fkd::k_batch_aux<S, B, L, KQ, MF>};
+}
+
+// Complex 2x2 blocks: pass widths 16, 32, 64 packets; complex 8x8 blocks:
+// matrix cores (tiles of 16 columns) with the outer products at width 16
+// (FP64) or 8 (FP32) packets.
+const std::vector<ClassDef>& classes(bool single) {
+  static const std::vector<ClassDef> d = {
+      mk_class<double, 2, 8, 2, false>(16), mk_class<double, 2, 16, 2, false>(32),
+      mk_class<double, 2, 64, 1, false>(64), mk_class<double, 8, 16, 1, true>(16)};
+  static const std::vector<ClassDef> f = {
+      mk_class<float, 2, 8, 2, false>(16), mk_class<float, 2, 16, 2, false>(32),
+      mk_class<float, 2, 64, 1, false>(64), mk_class<float, 8, 8, 1, true>(8)};
+  return single ? f : d;
+}
+
+// First-fit arena over one device allocation; offsets aligned to 256 bytes.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0, peak = 0;
+  std::map<size_t, size_t> holes;  // offset -> length
+  void init(char* b, size_t n) {
+    base = b;
+    size = n;
+    holes.clear();
+    holes[0] = n;
+  }
+  bool alloc(size_t n, size_t* off) {
+    n = (n + 255) & ~(size_t)255;
+    if (n == 0) n = 256;
+    for (auto it = holes.begin(); it != holes.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        const size_t rest = it->second - n, o = it->first + n;
+        holes.erase(it);
+        if (rest) holes[o] = rest;
+        used += n;
+        peak = std::max(peak, used);
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    if (n == 0) n = 256;
+    used -= n;
+    auto it = holes.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != holes.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      holes.erase(nx);
+    }
+    if (it != holes.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        holes.erase(it);
+      }
+    }
+  }
+};
+
+struct Request {
+  long long id = 0;
+  int nb = 0, b = 0, nnzb = 0, n = 0, s = 0, M = 0;
+  double lo = 0, hi = 0;
+  std::vector<int> rp, ci, rowblk, tpos;
+  std::vector<double> val, R, c;
+  bool compact = false;  // R given as (rcol, rval) per orbital row (R holds rval)
+  std::vector<int> rcol;
+  // results
+  std::vector<double> mu, D;
+  double energy = 0;
+  int status = 0;  // 0 queued, 1 running, 2 done, < 0 error code
+  std::string err;
+  Clock::time_point t_submit, t_admit, t_done;
+};
+
+// Per-request layout of the slot's buffers in the arena.
+struct Layout {
+  int cls = 0, W = 0, qp = 0, sp = 0, nwg = 0, chunk = 0;
+  size_t off[24];
+  size_t len[24];
+  int nbuf = 0;
+  size_t total = 0;
+};
+
+enum Buf { B_RP, B_CI, B_RB, B_TP, B_HV, B_HRAW, B_R, B_A0, B_A1, B_A2, B_B0, B_B1, B_B2, B_RRAW,
+           B_DACC, B_DOUT, B_PART, B_PSUM, B_C, B_COUNT };
+
+int pick_class(bool single, int b, int s, int* W) {
+  const auto& cl = classes(single);
+  const int ne = single ? 2 : 1;  // complex elements per 16-byte packet
+  const int need = (s + ne - 1) / ne;
+  int best = -1;
+  for (int k = 0; k < (int)cl.size(); k++) {
+    if (cl[k].B != b) continue;
+    if (best < 0) {
+      best = k;
+      continue;
+    }
+    // smallest width covering the row, else the widest
+    const bool cov_k = cl[k].W >= need, cov_b = cl[best].W >= need;
+    if ((cov_k && (!cov_b || cl[k].W < cl[best].W)) || (!cov_k && !cov_b && cl[k].W > cl[best].W))
+      best = k;
+  }
+  if (best >= 0) *W = cl[best].W;
+  return best;
+}
+
+void plan(bool single, const Request& r, Layout* L) {
+  const size_t es = single ? 4 : 8;
+  L->cls = pick_class(single, r.b, r.s, &L->W);
+  if (L->cls < 0) {
+    L->total = 0;
+    return;
+  }
+  const int ne = single ? 2 : 1;
+  const int need = (r.s + ne - 1) / ne;
+  L->qp = ((need + L->W - 1) / L->W) * L->W;
+  L->sp = L->qp * ne;
+  // workgroups per slot: one per 4 block rows up to 2048 (fixed by the shape)
+  int nwg = std::min(std::max((r.nb + 3) / 4, 1), 2048);
+  int chunk = (r.nb + nwg - 1) / nwg;
+  nwg = (r.nb + chunk - 1) / chunk;
+  L->nwg = nwg;
+  L->chunk = chunk;
+  const size_t vec = (size_t)r.n * L->sp * 2 * es;
+  const size_t dbl = (size_t)r.nnzb * r.b * r.b * 2 * sizeof(double);
+  const size_t h = r.M / 2;
+  size_t len[B_COUNT];
+  len[B_RP] = sizeof(int) * (r.nb + 1);
+  len[B_CI] = sizeof(int) * r.nnzb;
+  len[B_RB] = sizeof(int) * r.nnzb;
+  len[B_TP] = sizeof(int) * r.nnzb;
+  len[B_HV] = (size_t)r.nnzb * r.b * r.b * 2 * es;
+  len[B_HRAW] = dbl;
+  len[B_R] = len[B_A0] = len[B_A1] = len[B_A2] = len[B_B0] = len[B_B1] = len[B_B2] = vec;
+  len[B_RRAW] = r.compact ? (size_t)r.n * (sizeof(int) + 2 * sizeof(double)) + 256
+                         : (size_t)r.n * r.s * 2 * sizeof(double);
+  len[B_DACC] = len[B_DOUT] = dbl;
+  len[B_PART] = h * nwg * 2 * sizeof(double);
+  len[B_PSUM] = h * 2 * sizeof(double);
+  len[B_C] = sizeof(double) * r.M;
+  L->nbuf = B_COUNT;
+  L->total = 0;
+  for (int k = 0; k < B_COUNT; k++) {
+    L->len[k] = len[k];
+    L->total += (len[k] + 255) & ~(size_t)255;
+  }
+}
+
+struct Slot {
+  std::shared_ptr<Request> rq;
+  Layout lay;
+  int idx = -1;
+  bool staging_live = true;
+  SlotDev dev{};
+};
+
+}  // namespace
+
+struct fkpm_batch {
+  int device = 0;
+  bool single = false;
+  int max_slots = 0;
+  hipStream_t stream = nullptr;
+  char* arena_mem = nullptr;
+  Arena arena;
+  // device tables
+  SlotDev* d_slots = nullptr;
+  int* d_st = nullptr;
+  int* d_act = nullptr;
+  std::vector<int*> d_cls_slots, d_cls_off;
+  // host mirrors
+  std::vector<SlotDev> h_slots;
+  std::vector<int> h_act, h_st;
+  std::vector<std::vector<int>> h_cls_slots, h_cls_off;
+  std::vector<int> free_idx;
+  // scheduling
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> pending;
+  std::map<long long, std::shared_ptr<Request>> reqs;
+  std::vector<Slot> active;
+  long long next_id = 1;
+  bool stop = false;
+  std::string broken;  // sticky failure of the scheduler
+  std::thread worker;
+  // options and statistics
+  bool use_graph = true;
+  int max_ticks = 16;
+  long long n_ticks = 0, n_replays = 0, n_captures = 0;
+  int peak_slots = 0;
+  // graph cache for the current slot set
+  long long set_version = 0, graph_version = -1;
+  std::map<int, hipGraphExec_t> graphs;  // by tick count
+
+  void clear_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+};
+
+namespace {
+
+const int kZero = 0;
+
+#define BCHK(x)                                                                       \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) {                                                           \
+      char m_[256];                                                                   \
+      snprintf(m_, sizeof(m_), "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),    \
+               __FILE__, __LINE__);                                                   \
+      throw std::runtime_error(m_);                                                   \
+    }                                                                                 \
+  } while (0)
+
+void launch_k(BatchKernel k, int grid, hipStream_t st, const SlotDev* sd, const int* stp,
+              const int* slots, const int* off, int ns) {
+  void* argv[] = {(void*)&sd, (void*)&stp, (void*)&slots, (void*)&off, (void*)&ns};
+  BCHK(hipLaunchKernel((const void*)k, dim3(grid), dim3(fkd::WG), argv, 0, st));
+}
+
+// Upload a request into its slot's buffers and prepare H, R on the device.
+void admit_one(fkpm_batch* B, Slot& sl) {
+  Request& r = *sl.rq;
+  const Layout& L = sl.lay;
+  char* base = B->arena.base;
+  auto P = [&](int k) { return (void*)(base + L.off[k]); };
+  hipStream_t st = B->stream;
+  BCHK(hipMemcpyAsync(P(B_RP), r.rp.data(), L.len[B_RP], hipMemcpyHostToDevice, st));
+  BCHK(hipMemcpyAsync(P(B_CI), r.ci.data(), L.len[B_CI], hipMemcpyHostToDevice, st));
+  BCHK(hipMemcpyAsync(P(B_RB), r.rowblk.data(), L.len[B_RB], hipMemcpyHostToDevice, st));
+  BCHK(hipMemcpyAsync(P(B_TP), r.tpos.data(), L.len[B_TP], hipMemcpyHostToDevice, st));
+  BCHK(hipMemcpyAsync(P(B_HRAW), r.val.data(), L.len[B_HRAW], hipMemcpyHostToDevice, st));
+  // compact R: rcol then rval (16-byte aligned) in the staging buffer
+  const size_t rv_off = ((size_t)r.n * sizeof(int) + 255) & ~(size_t)255;
+  if (r.compact) {
+    BCHK(hipMemcpyAsync(P(B_RRAW), r.rcol.data(), sizeof(int) * r.n, hipMemcpyHostToDevice, st));
+    BCHK(hipMemcpyAsync((char*)P(B_RRAW) + rv_off, r.R.data(), sizeof(double) * 2 * r.n,
+                        hipMemcpyHostToDevice, st));
+  } else {
+    BCHK(hipMemcpyAsync(P(B_RRAW), r.R.data(), L.len[B_RRAW], hipMemcpyHostToDevice, st));
+  }
+  BCHK(hipMemcpyAsync(P(B_C), r.c.data(), L.len[B_C], hipMemcpyHostToDevice, st));
+  const double avg = (r.hi + r.lo) / 2, mag = (r.hi - r.lo) / 2;
+  if (fkpm_internal::scale_H_cx(st, (const double*)P(B_HRAW), P(B_HV), B->single,
+                                (const int*)P(B_RB), (const int*)P(B_CI), r.nnzb, r.b, mag,
+                                avg / mag) != 0)
+    throw std::runtime_error(fkpm_last_error());
+  if (r.compact) {
+    const int* rc = (const int*)P(B_RRAW);
+    const double* rv = (const double*)((char*)P(B_RRAW) + rv_off);
+    const size_t tot = (size_t)r.n * L.sp;
+    const int grid = (int)std::min<size_t>((tot + fkd::WG - 1) / fkd::WG, 65536);
+    const int n = r.n, sp = L.sp;
+    if (B->single) {
+      float* Rd = (float*)P(B_R);
+      void* argv[] = {(void*)&rc, (void*)&rv, (void*)&n, (void*)&sp, (void*)&Rd};
+      BCHK(hipLaunchKernel((const void*)&fkd::k_batch_scatter_R<float>, dim3(grid), dim3(fkd::WG),
+                           argv, 0, st));
+    } else {
+      double* Rd = (double*)P(B_R);
+      void* argv[] = {(void*)&rc, (void*)&rv, (void*)&n, (void*)&sp, (void*)&Rd};
+      BCHK(hipLaunchKernel((const void*)&fkd::k_batch_scatter_R<double>, dim3(grid),
+                           dim3(fkd::WG), argv, 0, st));
+    }
+  } else if (fkpm_internal::pack_R_cx(st, (const double*)P(B_RRAW), P(B_R), B->single, r.n, r.s,
+                                      L.sp) != 0) {
+    throw std::runtime_error(fkpm_last_error());
+  }
+  SlotDev& d = sl.dev;
+  std::memset(&d, 0, sizeof(d));
+  d.rp = (const int*)P(B_RP);
+  d.ci = (const int*)P(B_CI);
+  d.hv = P(B_HV);
+  d.tpos = (const int*)P(B_TP);
+  d.R = P(B_R);
+  d.A[0] = P(B_A0);
+  d.A[1] = P(B_A1);
+  d.A[2] = P(B_A2);
+  d.Bv[0] = P(B_B0);
+  d.Bv[1] = P(B_B1);
+  d.Bv[2] = P(B_B2);
+  d.dacc = (double*)P(B_DACC);
+  d.dout = (double*)P(B_DOUT);
+  d.partial = (double*)P(B_PART);
+  d.psum = (double*)P(B_PSUM);
+  d.c = (const double*)P(B_C);
+  d.nb = r.nb;
+  d.qp = L.qp;
+  d.chunk = L.chunk;
+  d.nwg = L.nwg;
+  d.M = r.M;
+  d.inv_mag = 1.0 / mag;
+  double diag = r.c[1];
+  for (int m = 1; m < r.M / 2; m++) diag -= r.c[2 * m + 1];
+  d.diag = diag;
+  d.sb = 2 * r.c[r.M - 1];
+  B->h_slots[sl.idx] = d;
+  BCHK(hipMemcpyAsync(B->d_slots + sl.idx, &B->h_slots[sl.idx], sizeof(SlotDev),
+                      hipMemcpyHostToDevice, st));
+  BCHK(hipMemcpyAsync(B->d_st + sl.idx, &kZero, sizeof(int), hipMemcpyHostToDevice, st));
+}
+
+void release_staging(fkpm_batch* B, Slot& sl) {
+  if (!sl.staging_live) return;
+  B->arena.release(sl.lay.off[B_HRAW], sl.lay.len[B_HRAW]);
+  B->arena.release(sl.lay.off[B_RRAW], sl.lay.len[B_RRAW]);
+  B->arena.release(sl.lay.off[B_RB], sl.lay.len[B_RB]);
+  sl.lay.len[B_HRAW] = sl.lay.len[B_RRAW] = sl.lay.len[B_RB] = 0;
+  sl.staging_live = false;
+}
+
+void release_slot(fkpm_batch* B, Slot& sl) {
+  release_staging(B, sl);
+  for (int k = 0; k < B_COUNT; k++)
+    if (k != B_HRAW && k != B_RRAW && k != B_RB) B->arena.release(sl.lay.off[k], sl.lay.len[k]);
+  B->free_idx.push_back(sl.idx);
+}
+
+// Try to allocate every buffer of a request; all or nothing.
+bool try_alloc(fkpm_batch* B, Layout& L) {
+  int k = 0;
+  for (; k < B_COUNT; k++)
+    if (!B->arena.alloc(L.len[k], &L.off[k])) break;
+  if (k == B_COUNT) return true;
+  for (int j = 0; j < k; j++) B->arena.release(L.off[j], L.len[j]);
+  return false;
+}
+
+// Per-class slot lists, workgroup prefix tables and the active list.
+void upload_tables(fkpm_batch* B) {
+  const auto& cl = classes(B->single);
+  const size_t nc = cl.size();
+  for (size_t c = 0; c < nc; c++) {
+    B->h_cls_slots[c].clear();
+    B->h_cls_off[c].assign(1, 0);
+  }
+  B->h_act.clear();
+  for (const Slot& sl : B->active) {
+    const int c = sl.lay.cls;
+    B->h_cls_slots[c].push_back(sl.idx);
+    B->h_cls_off[c].push_back(B->h_cls_off[c].back() + sl.lay.nwg);
+    B->h_act.push_back(sl.idx);
+  }
+  for (size_t c = 0; c < nc; c++) {
+    if (B->h_cls_slots[c].empty()) continue;
+    BCHK(hipMemcpyAsync(B->d_cls_slots[c], B->h_cls_slots[c].data(),
+                        sizeof(int) * B->h_cls_slots[c].size(), hipMemcpyHostToDevice, B->stream));
+    BCHK(hipMemcpyAsync(B->d_cls_off[c], B->h_cls_off[c].data(),
+                        sizeof(int) * B->h_cls_off[c].size(), hipMemcpyHostToDevice, B->stream));
+  }
+  if (!B->h_act.empty())
+    BCHK(hipMemcpyAsync(B->d_act, B->h_act.data(), sizeof(int) * B->h_act.size(),
+                        hipMemcpyHostToDevice, B->stream));
+  BCHK(hipStreamSynchronize(B->stream));  // host tables are reused next time
+}
+
+void enqueue_ticks(fkpm_batch* B, int T) {
+  const auto& cl = classes(B->single);
+  const int na = (int)B->h_act.size();
+  for (int t = 0; t < T; t++) {
+    for (size_t c = 0; c < cl.size(); c++) {
+      const int ns = (int)B->h_cls_slots[c].size();
+      if (ns == 0) continue;
+      const int grid = B->h_cls_off[c].back();
+      launch_k(cl[c].fwd, grid, B->stream, B->d_slots, B->d_st, B->d_cls_slots[c],
+               B->d_cls_off[c], ns);
+      launch_k(cl[c].bwd, grid, B->stream, B->d_slots, B->d_st, B->d_cls_slots[c],
+               B->d_cls_off[c], ns);
+      launch_k(cl[c].aux, grid, B->stream, B->d_slots, B->d_st, B->d_cls_slots[c],
+               B->d_cls_off[c], ns);
+    }
+    int* st = B->d_st;
+    const int* act = B->d_act;
+    void* argv[] = {(void*)&st, (void*)&act, (void*)&na};
+    BCHK(hipLaunchKernel((const void*)&fkd::k_batch_advance, dim3((na + 255) / 256), dim3(256),
+                         argv, 0, B->stream));
+  }
+}
+
+void run_ticks(fkpm_batch* B, int T) {
+  if (!B->use_graph) {
+    enqueue_ticks(B, T);
+  } else {
+    if (B->graph_version != B->set_version) {
+      B->clear_graphs();
+      B->graph_version = B->set_version;
+    }
+    auto it = B->graphs.find(T);
+    if (it == B->graphs.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ex = nullptr;
+      BCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue_ticks(B, T);
+      } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(B->stream, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+      }
+      BCHK(hipStreamEndCapture(B->stream, &g));
+      BCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      it = B->graphs.emplace(T, ex).first;
+      B->n_captures++;
+    }
+    BCHK(hipGraphLaunch(it->second, B->stream));
+  }
+  B->n_ticks += T;
+  B->n_replays++;
+}
+
+// Moments and D of a finished slot to its request.
+void retire(fkpm_batch* B, Slot& sl) {
+  Request& r = *sl.rq;
+  const int h = r.M / 2;
+  std::vector<double> ps((size_t)h * 2);
+  r.D.resize((size_t)r.nnzb * r.b * r.b * 2);
+  BCHK(hipMemcpyAsync(ps.data(), sl.dev.psum, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
+                      B->stream));
+  BCHK(hipMemcpyAsync(r.D.data(), sl.dev.dout, r.D.size() * sizeof(double), hipMemcpyDeviceToHost,
+                      B->stream));
+  BCHK(hipStreamSynchronize(B->stream));
+  r.mu.assign(r.M, 0.0);
+  r.mu[0] = ps[0];
+  r.mu[1] = ps[1];
+  for (int m = 1; m < h; m++) {
+    r.mu[2 * m] = 2 * ps[2 * m] - r.mu[0];
+    r.mu[2 * m + 1] = 2 * ps[2 * m + 1] - r.mu[1];
+  }
+  double e = 0.0;
+  for (int m = 0; m < r.M; m++) e += r.c[m] * r.mu[m];
+  r.energy = e;
+  // inputs are no longer needed
+  std::vector<double>().swap(r.val);
+  std::vector<double>().swap(r.R);
+}
+
+void fail_all(fkpm_batch* B, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(B->mu);
+  B->broken = msg;
+  for (Slot& sl : B->active) {
+    sl.rq->status = FKPM_E_HIP;
+    sl.rq->err = msg;
+  }
+  B->active.clear();
+  for (auto& r : B->pending) {
+    r->status = FKPM_E_HIP;
+    r->err = msg;
+  }
+  B->pending.clear();
+  B->cv_done.notify_all();
+}
+
+void scheduler(fkpm_batch* B) {
+  try {
+    BCHK(hipSetDevice(B->device));
+    for (;;) {
+      std::vector<size_t> fresh;  // positions in `active` of slots admitted now
+      {
+        std::unique_lock<std::mutex> lk(B->mu);
+        B->cv_work.wait(lk, [&] { return B->stop || !B->pending.empty() || !B->active.empty(); });
+        if (B->stop) return;
+        // admission: FIFO while a slot is free and the footprint fits
+        while (!B->pending.empty() && (int)B->active.size() < B->max_slots) {
+          std::shared_ptr<Request> r = B->pending.front();
+          Slot sl;
+          plan(B->single, *r, &sl.lay);
+          if (!try_alloc(B, sl.lay)) {
+            if (B->active.empty()) {  // cannot fit even alone: refuse it
+              B->pending.pop_front();
+              r->status = FKPM_E_ARG;
+              r->err = "request does not fit the batch's HBM budget";
+              B->cv_done.notify_all();
+              continue;
+            }
+            break;
+          }
+          B->pending.pop_front();
+          sl.rq = r;
+          sl.idx = B->free_idx.back();
+          B->free_idx.pop_back();
+          B->h_st[sl.idx] = 0;
+          r->status = 1;
+          r->t_admit = Clock::now();
+          fresh.push_back(B->active.size());
+          B->active.push_back(sl);
+        }
+        B->peak_slots = std::max(B->peak_slots, (int)B->active.size());
+      }
+      if (!fresh.empty()) {
+        for (size_t k : fresh) admit_one(B, B->active[k]);
+        upload_tables(B);
+        B->set_version++;
+      }
+      if (B->active.empty()) continue;
+      // ticks until the first slot finishes (at most max_ticks)
+      int T = B->max_ticks;
+      for (const Slot& sl : B->active) {
+        const int left = sl.rq->M + 2 - B->h_st[sl.idx];
+        T = std::min(T, std::max(left, 1));
+      }
+      run_ticks(B, T);
+      BCHK(hipStreamSynchronize(B->stream));
+      // new slots' staging buffers were consumed by their first ticks
+      for (Slot& sl : B->active) release_staging(B, sl);
+      BCHK(hipMemcpy(B->h_st.data(), B->d_st, sizeof(int) * B->max_slots, hipMemcpyDeviceToHost));
+      std::vector<Slot> keep;
+      std::vector<Slot> done;
+      for (Slot& sl : B->active) {
+        if (B->h_st[sl.idx] >= sl.rq->M + 2) done.push_back(sl); else keep.push_back(sl);
+      }
+      for (Slot& sl : done) retire(B, sl);
+      if (!done.empty()) {
+        B->active.swap(keep);
+        upload_tables(B);
+        B->set_version++;
+        std::lock_guard<std::mutex> lk(B->mu);
+        const auto now = Clock::now();
+        for (Slot& sl : done) {
+          sl.rq->t_done = now;
+          sl.rq->status = 2;
+          release_slot(B, sl);
+        }
+        B->cv_done.notify_all();
+      }
+    }
+  } catch (const std::exception& e) {
+    fail_all(B, e.what());
+  }
+}
+
+int berr(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  return fkpm_internal::set_error(code, buf);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fkpm_batch_footprint(int prec, int nb, int b, int nnzb, int s, int M, size_t* bytes) {
+  if (!bytes || nb <= 0 || nnzb < nb || s <= 0 || M < 2 || (b != 2 && b != 8))
+    return berr(FKPM_E_ARG, "batch_footprint: bad arguments");
+  Request r;
+  r.nb = nb;
+  r.b = b;
+  r.nnzb = nnzb;
+  r.n = nb * b;
+  r.s = s;
+  r.M = M;
+  Layout L;
+  plan(prec == FKPM_PREC_FP32, r, &L);
+  *bytes = L.total;
+  return 0;
+}
+
+int fkpm_batch_open(int device, int prec, size_t hbm_bytes, int max_slots, fkpm_batch** out) {
+  if (!out || max_slots < 1 || max_slots > 4096 || hbm_bytes < (1u << 20))
+    return berr(FKPM_E_ARG, "batch_open: bad arguments");
+  if (prec != FKPM_PREC_FP64 && prec != FKPM_PREC_FP32)
+    return berr(FKPM_E_ARG, "batch_open: bad precision %d", prec);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return berr(FKPM_E_NODEV, "no HIP device");
+  if (device < 0 || device >= ndev) return berr(FKPM_E_ARG, "bad device %d", device);
+  std::unique_ptr<fkpm_batch> B(new fkpm_batch());
+  B->device = device;
+  B->single = prec == FKPM_PREC_FP32;
+  B->max_slots = max_slots;
+  try {
+    BCHK(hipSetDevice(device));
+    BCHK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    BCHK(hipMalloc((void**)&B->arena_mem, hbm_bytes));
+    B->arena.init(B->arena_mem, hbm_bytes);
+    BCHK(hipMalloc((void**)&B->d_slots, sizeof(SlotDev) * max_slots));
+    BCHK(hipMalloc((void**)&B->d_st, sizeof(int) * max_slots));
+    BCHK(hipMalloc((void**)&B->d_act, sizeof(int) * max_slots));
+    const size_t nc = classes(B->single).size();
+    B->d_cls_slots.assign(nc, nullptr);
+    B->d_cls_off.assign(nc, nullptr);
+    for (size_t c = 0; c < nc; c++) {
+      BCHK(hipMalloc((void**)&B->d_cls_slots[c], sizeof(int) * max_slots));
+      BCHK(hipMalloc((void**)&B->d_cls_off[c], sizeof(int) * (max_slots + 1)));
+    }
+    B->h_cls_slots.assign(nc, {});
+    B->h_cls_off.assign(nc, {});
+    B->h_slots.assign(max_slots, SlotDev());
+    B->h_st.assign(max_slots, 0);
+    std::vector<int> zeros(max_slots, 0);
+    BCHK(hipMemcpy(B->d_st, zeros.data(), sizeof(int) * max_slots, hipMemcpyHostToDevice));
+    for (int k = max_slots - 1; k >= 0; k--) B->free_idx.push_back(k);
+  } catch (const std::exception& e) {
+    return berr(FKPM_E_HIP, "%s", e.what());
+  }
+  B->worker = std::thread(scheduler, B.get());
+  *out = B.release();
+  return 0;
+}
+
+int fkpm_batch_close(fkpm_batch* B) {
+  if (!B) return berr(FKPM_E_ARG, "null batch");
+  {
+    std::lock_guard<std::mutex> lk(B->mu);
+    B->stop = true;
+  }
+  B->cv_work.notify_all();
+  if (B->worker.joinable()) B->worker.join();
+  (void)hipSetDevice(B->device);
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  B->clear_graphs();
+  for (size_t c = 0; c < B->d_cls_slots.size(); c++) {
+    (void)hipFree(B->d_cls_slots[c]);
+    (void)hipFree(B->d_cls_off[c]);
+  }
+  (void)hipFree(B->d_slots);
+  (void)hipFree(B->d_st);
+  (void)hipFree(B->d_act);
+  (void)hipFree(B->arena_mem);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+  return 0;
+}
+
+int fkpm_batch_set_option(fkpm_batch* B, int option, int value) {
+  if (!B) return berr(FKPM_E_ARG, "null batch");
+  std::lock_guard<std::mutex> lk(B->mu);
+  switch (option) {
+    case FKPM_BATCH_OPT_GRAPH:
+      B->use_graph = value != 0;
+      return 0;
+    case FKPM_BATCH_OPT_TICKS:
+      if (value < 1 || value > 4096) return berr(FKPM_E_ARG, "bad tick count %d", value);
+      B->max_ticks = value;
+      return 0;
+  }
+  return berr(FKPM_E_ARG, "unknown batch option %d", option);
+}
+
+static int submit_common(fkpm_batch* B, int nb, int b, int nnzb, const int* row_ptr,
+                         const int* col_idx, const void* val, double lo, double hi, int n, int s,
+                         const void* R_colmajor, const int* rcol, const void* rval, int M,
+                         const double* c, long long* id) {
+  if (!B || !row_ptr || !col_idx || !val || !(R_colmajor || (rcol && rval)) || !c || !id)
+    return berr(FKPM_E_ARG, "batch_submit: null argument");
+  if (b != 2 && b != 8) return berr(FKPM_E_UNSUP, "batch: block size %d (2 or 8 supported)", b);
+  if (nb <= 0 || nnzb < nb || n != nb * b || s <= 0 || M < 2 || M % 2 != 0 || !(hi > lo))
+    return berr(FKPM_E_ARG, "batch_submit: bad sizes (nb %d, b %d, n %d, s %d, M %d)", nb, b, n,
+                s, M);
+  auto r = std::make_shared<Request>();
+  r->nb = nb;
+  r->b = b;
+  r->nnzb = nnzb;
+  r->n = n;
+  r->s = s;
+  r->M = M;
+  r->lo = lo;
+  r->hi = hi;
+  r->rp.assign(row_ptr, row_ptr + nb + 1);
+  r->ci.assign(col_idx, col_idx + nnzb);
+  if (r->rp[0] != 0 || r->rp[nb] != nnzb) return berr(FKPM_E_ARG, "batch_submit: bad row_ptr");
+  r->rowblk.resize(nnzb);
+  for (int i = 0; i < nb; i++) {
+    if (r->rp[i + 1] < r->rp[i]) return berr(FKPM_E_ARG, "batch_submit: row_ptr not monotone");
+    bool diag = false;
+    for (int p = r->rp[i]; p < r->rp[i + 1]; p++) {
+      const int j = r->ci[p];
+      if (j < 0 || j >= nb || (p > r->rp[i] && j <= r->ci[p - 1]))
+        return berr(FKPM_E_ARG, "batch_submit: column indices of row %d unsorted/out of range", i);
+      diag |= (j == i);
+      r->rowblk[p] = i;
+    }
+    if (!diag) return berr(FKPM_E_ARG, "batch_submit: diagonal block %d missing", i);
+  }
+  // transpose positions (structurally symmetric pattern, as symmetrize needs)
+  r->tpos.assign(nnzb, -1);
+  for (int i = 0; i < nb; i++)
+    for (int p = r->rp[i]; p < r->rp[i + 1]; p++) {
+      const int j = r->ci[p];
+      const int* b0 = r->ci.data() + r->rp[j];
+      const int* b1 = r->ci.data() + r->rp[j + 1];
+      const int* f = std::lower_bound(b0, b1, i);
+      if (f == b1 || *f != i) return berr(FKPM_E_ARG, "batch_submit: pattern not symmetric");
+      r->tpos[p] = (int)(f - r->ci.data());
+    }
+  const size_t hvn = (size_t)nnzb * b * b * 2;
+  r->val.assign((const double*)val, (const double*)val + hvn);
+  if (R_colmajor) {
+    r->R.assign((const double*)R_colmajor, (const double*)R_colmajor + (size_t)n * s * 2);
+  } else {
+    r->compact = true;
+    r->rcol.assign(rcol, rcol + n);
+    for (int i = 0; i < n; i++)
+      if (r->rcol[i] < -1 || r->rcol[i] >= s)
+        return berr(FKPM_E_ARG, "batch_submit: probing column %d of row %d out of range",
+                    r->rcol[i], i);
+    r->R.assign((const double*)rval, (const double*)rval + (size_t)n * 2);
+  }
+  r->c.assign(c, c + M);
+  Layout L;
+  plan(B->single, *r, &L);
+  if (L.cls < 0) return berr(FKPM_E_UNSUP, "batch: no kernel class for block %d", b);
+  std::lock_guard<std::mutex> lk(B->mu);
+  if (!B->broken.empty()) return berr(FKPM_E_HIP, "batch engine failed: %s", B->broken.c_str());
+  if (L.total > B->arena.size)
+    return berr(FKPM_E_ARG, "request needs %zu bytes, more than the batch's HBM budget %zu",
+                L.total, B->arena.size);
+  r->id = B->next_id++;
+  r->t_submit = Clock::now();
+  B->pending.push_back(r);
+  B->reqs[r->id] = r;
+  *id = r->id;
+  B->cv_work.notify_one();
+  return 0;
+}
+
+int fkpm_batch_submit(fkpm_batch* B, int nb, int b, int nnzb, const int* row_ptr,
+                      const int* col_idx, const void* val, double lo, double hi, int n, int s,
+                      const void* R_colmajor, int M, const double* c, long long* id) {
+  return submit_common(B, nb, b, nnzb, row_ptr, col_idx, val, lo, hi, n, s, R_colmajor, nullptr,
+                       nullptr, M, c, id);
+}
+
+int fkpm_batch_submit_probing(fkpm_batch* B, int nb, int b, int nnzb, const int* row_ptr,
+                              const int* col_idx, const void* val, double lo, double hi, int n,
+                              int s, const int* rcol, const void* rval, int M, const double* c,
+                              long long* id) {
+  if (!rcol || !rval) return berr(FKPM_E_ARG, "batch_submit_probing: null probing vectors");
+  return submit_common(B, nb, b, nnzb, row_ptr, col_idx, val, lo, hi, n, s, nullptr, rcol, rval,
+                       M, c, id);
+}
+
+int fkpm_batch_poll(fkpm_batch* B, long long id, int* done) {
+  if (!B || !done) return berr(FKPM_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(B->mu);
+  auto it = B->reqs.find(id);
+  if (it == B->reqs.end()) return berr(FKPM_E_ARG, "unknown request %lld", id);
+  *done = it->second->status == 2 || it->second->status < 0 ? 1 : 0;
+  return 0;
+}
+
+int fkpm_batch_wait(fkpm_batch* B, long long id, double* mu, double* energy, double* D_val,
+                    double* times_ms) {
+  if (!B) return berr(FKPM_E_ARG, "null batch");
+  std::shared_ptr<Request> r;
+  {
+    std::unique_lock<std::mutex> lk(B->mu);
+    auto it = B->reqs.find(id);
+    if (it == B->reqs.end()) return berr(FKPM_E_ARG, "unknown request %lld", id);
+    r = it->second;
+    B->cv_done.wait(lk, [&] { return r->status == 2 || r->status < 0; });
+    B->reqs.erase(it);
+  }
+  if (r->status < 0) return berr(r->status, "request %lld failed: %s", id, r->err.c_str());
+  if (mu) std::copy(r->mu.begin(), r->mu.end(), mu);
+  if (energy) *energy = r->energy;
+  if (D_val) std::copy(r->D.begin(), r->D.end(), D_val);
+  if (times_ms) {
+    auto ms = [](Clock::duration d) { return std::chrono::duration<double, std::milli>(d).count(); };
+    times_ms[0] = ms(r->t_admit - r->t_submit);
+    times_ms[1] = ms(r->t_done - r->t_admit);
+    times_ms[2] = ms(r->t_done - r->t_submit);
+  }
+  return 0;
+}
+
+int fkpm_batch_stats(fkpm_batch* B, long long* ticks, long long* replays, long long* captures,
+                     int* peak_slots, size_t* peak_bytes) {
+  if (!B) return berr(FKPM_E_ARG, "null batch");
+  std::lock_guard<std::mutex> lk(B->mu);
+  if (ticks) *ticks = B->n_ticks;
+  if (replays) *replays = B->n_replays;
+  if (captures) *captures = B->n_captures;
+  if (peak_slots) *peak_slots = B->peak_slots;
+  if (peak_bytes) *peak_bytes = B->arena.peak;
+  return 0;
+}
+
+}  // extern "C"

@@ -21,6 +21,7 @@
 
 #include "fastkpm_hip.h"
 #include "kpm_dispatch.h"
+#include "kpm_internal.h"
 // this unit defines the non-template helper kernels
 #define FKD_DEFINE_HELPERS
 #include "kpm_kernels.h"
@@ -1430,3 +1431,32 @@ int geometries_for_block(int b, GeomChoice* out, int cap) {
   return n;
 }
 }  // namespace fkd
+
+// ---------------------------------------------------------------------------
+// Host functions for the batched engine (kpm_batch.hip).
+// ---------------------------------------------------------------------------
+namespace fkpm_internal {
+
+int scale_H_cx(hipStream_t st, const double* raw, void* dst, bool single, const int* row_of_block,
+               const int* ci, int nnzb, int b, double mag, double shift) {
+  const size_t cnt = (size_t)nnzb * b * b;
+  if (cnt == 0) return 0;
+  if (single)
+    return launch(&fkd::k_scale_H<double, float, true>, dim3(grid_for(cnt)), dim3(256), st, raw,
+                  (float*)dst, row_of_block, ci, nnzb, b, mag, shift);
+  return launch(&fkd::k_scale_H<double, double, true>, dim3(grid_for(cnt)), dim3(256), st, raw,
+                (double*)dst, row_of_block, ci, nnzb, b, mag, shift);
+}
+
+int pack_R_cx(hipStream_t st, const double* raw, void* dst, bool single, int n, int s, int sp) {
+  const dim3 grid((n + 63) / 64, (sp + 15) / 16);
+  if (single)
+    return launch(&fkd::k_pack_R<double, float, true>, grid, dim3(fkd::WG), st, raw, (float*)dst, n,
+                  s, sp);
+  return launch(&fkd::k_pack_R<double, double, true>, grid, dim3(fkd::WG), st, raw, (double*)dst,
+                n, s, sp);
+}
+
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+}  // namespace fkpm_internal

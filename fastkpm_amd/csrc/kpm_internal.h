@@ -1,0 +1,20 @@
+// kpm_internal.h -- host functions shared between the translation units of
+// libfastkpm_hip.so (not exported).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fkpm_internal {
+
+// Hs = (H - shift I) / mag from complex-double host-layout values `raw` (device)
+// into `dst` (FP32 or FP64 interleaved complex); row_of_block[p] = block row of p.
+int scale_H_cx(hipStream_t st, const double* raw, void* dst, bool single, const int* row_of_block,
+               const int* ci, int nnzb, int b, double mag, double shift);
+
+// R (n x s column-major complex double, device) -> n x sp row-major packed
+// (FP32 or FP64), padded columns zero.
+int pack_R_cx(hipStream_t st, const double* raw, void* dst, bool single, int n, int s, int sp);
+
+// Record an error message for fkpm_last_error() and return `code`.
+int set_error(int code, const char* msg);
+
+}  // namespace fkpm_internal

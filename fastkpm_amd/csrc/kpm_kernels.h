@@ -278,8 +278,16 @@ struct RowRange {
   int r0, r1, stride;
 };
 
-__device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step) {
-  const int k = blockIdx.x, n = gridDim.x;
+// A workgroup's index k among the n workgroups that share one sweep: the
+// launch's own (blockIdx, gridDim), or a sub-range of a batched launch.
+struct WgId {
+  int k, n;
+};
+
+__device__ __forceinline__ WgId wg_self() { return WgId{(int)blockIdx.x, (int)gridDim.x}; }
+
+__device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step, WgId wid) {
+  const int k = wid.k, n = wid.n;
   const int x = k & 7, r = k >> 3;
   const int per = n >> 3, extra = n & 7;
   const int nx = per + (x < extra ? 1 : 0);        // workgroups of XCD x
@@ -411,7 +419,8 @@ __device__ __forceinline__ void add_rsource(const RSource<S>& rs_, size_t row, s
 // step b_k = c_k R + 2 Hs b_{k+1} - b_{k+2} of the Clenshaw recurrence.
 // ---------------------------------------------------------------------------
 template <typename S, bool CX, int B, int L, int KQ>
-__global__ __launch_bounds__(WG) void k_step_moments(
+__device__ __forceinline__ void step_moments_dev(
+    WgId wid,
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
     int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
@@ -421,7 +430,7 @@ __global__ __launch_bounds__(WG) void k_step_moments(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG, wid);
   const size_t rs = (size_t)qp * NS;  // scalars per orbital row
   const A acx = (A)cx, acz = (A)cz;
   double dmm = 0.0, dm1 = 0.0;
@@ -471,7 +480,15 @@ __global__ __launch_bounds__(WG) void k_step_moments(
       }
     }
   }
-  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)blockIdx.x);
+  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
+}
+
+template <typename S, bool CX, int B, int L, int KQ>
+__global__ __launch_bounds__(WG) void k_step_moments(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
+  step_moments_dev<S, CX, B, L, KQ>(wg_self(), rp, ci, hv, x, z, y, nb, qp, chunk, cx, cz, partial, nt, rsrc);
 }
 
 // ---------------------------------------------------------------------------
@@ -491,7 +508,7 @@ __global__ __launch_bounds__(WG) void k_step_xi(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG, wg_self());
   const size_t rs = (size_t)qp * NS;
   const A acx = (A)cx, acz = (A)cz;
   const S* const X[1] = {x};
@@ -555,7 +572,8 @@ __global__ __launch_bounds__(WG) void k_step_xi(
 // means beta_{m+2} = sb2 * alpha_{m+1} (sb2 = 0 at the first step).
 // ---------------------------------------------------------------------------
 template <typename S, bool CX, int B, int L, int KQ, bool FUSED_D>
-__global__ __launch_bounds__(WG) void k_step_autodiff(
+__device__ __forceinline__ void step_autodiff_dev(
+    WgId wid,
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
     const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
@@ -568,7 +586,7 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG, wid);
   const size_t rs = (size_t)qp * NS;
   const bool vb1 = (bm1 == nullptr);
   const double hb1 = vb1 ? sb : 1.0;  // scale of the beta_{m+1} gather
@@ -702,6 +720,16 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
   }
 }
 
+template <typename S, bool CX, int B, int L, int KQ, bool FUSED_D>
+__global__ __launch_bounds__(WG) void k_step_autodiff(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  step_autodiff_dev<S, CX, B, L, KQ, FUSED_D>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out, dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
+}
+
 // ---------------------------------------------------------------------------
 // Sparse outer product on D's block pattern:
 //   D_p(bi, bj) += f * sum_s A[B i + bi, s] conj(Bm[B j + bj, s])
@@ -711,7 +739,8 @@ __global__ __launch_bounds__(WG) void k_step_autodiff(
 // TA / TB are the scalar types of A and Bm (storage type or double for xi).
 // ---------------------------------------------------------------------------
 template <typename S, typename TA, typename TB, bool CX, int B, int L, int KQ>
-__global__ __launch_bounds__(WG) void k_outer(
+__device__ __forceinline__ void outer_dev(
+    WgId wid,
     const int* __restrict__ drp, const int* __restrict__ dci, const TA* __restrict__ A,
     const TB* __restrict__ Bm, double* __restrict__ dacc, int nb, int qp, int chunk, double f) {
   using GE = Geo<S, CX, B, L, KQ>;
@@ -720,7 +749,7 @@ __global__ __launch_bounds__(WG) void k_outer(
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane % L;
   const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
-  const RowRange rr = wg_rows(nb, chunk, GE::GWG);
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG, wid);
   const size_t rs = (size_t)qp * NS;  // scalars per row (same count for every type)
 
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
@@ -803,6 +832,13 @@ __global__ __launch_bounds__(WG) void k_outer(
       }
     }
   }
+}
+
+template <typename S, typename TA, typename TB, bool CX, int B, int L, int KQ>
+__global__ __launch_bounds__(WG) void k_outer(
+    const int* __restrict__ drp, const int* __restrict__ dci, const TA* __restrict__ A,
+    const TB* __restrict__ Bm, double* __restrict__ dacc, int nb, int qp, int chunk, double f) {
+  outer_dev<S, TA, TB, CX, B, L, KQ>(wg_self(), drp, dci, A, Bm, dacc, nb, qp, chunk, f);
 }
 
 // ---------------------------------------------------------------------------

@@ -184,7 +184,8 @@ __device__ __forceinline__ void mfma_rsource(const RSource<S>& rs_, size_t row, 
 // Same contract as k_step_moments (engine_cpu.cpp:63-84 fused with both cdot's).
 // ---------------------------------------------------------------------------
 template <typename S>
-__global__ __launch_bounds__(WG) void k_mfma_step(
+__device__ __forceinline__ void mfma_step_dev(
+    WgId wid,
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
     int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(WG) void k_mfma_step(
   (void)nt;
   const MfmaLane m = mfma_lane<S>();
   const int g = threadIdx.x / WAVE;
-  const RowRange rr = wg_rows(nb, chunk, WG / WAVE);
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
   const size_t rs = (size_t)qp * NS;  // scalars per orbital row
   const int ntile = (int)(rs / 32);   // 16 complex columns per tile
   const S acx = (S)cx, acz = (S)cz;
@@ -245,7 +246,15 @@ __global__ __launch_bounds__(WG) void k_mfma_step(
       }
     }
   }
-  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)blockIdx.x);
+  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_mfma_step(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
+  mfma_step_dev<S>(wg_self(), rp, ci, hv, x, z, y, nb, qp, chunk, cx, cz, partial, nt, rsrc);
 }
 
 // Neighbour blocks of one pass of the autodiff kernel: the FP64 G
@@ -265,7 +274,8 @@ constexpr int MF_DMAX = 8;
 // parked in a_out / b_out, which only this wave touches.
 // ---------------------------------------------------------------------------
 template <typename S, bool FUSED_D>
-__global__ __launch_bounds__(WG) void k_mfma_autodiff(
+__device__ __forceinline__ void mfma_autodiff_dev(
+    WgId wid,
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
     const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
@@ -278,7 +288,7 @@ __global__ __launch_bounds__(WG) void k_mfma_autodiff(
   (void)nt;
   const MfmaLane m = mfma_lane<S>();
   const int g = threadIdx.x / WAVE;
-  const RowRange rr = wg_rows(nb, chunk, WG / WAVE);
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
   const size_t rs = (size_t)qp * NS;
   const int ntile = (int)(rs / 32);
   const bool vb1 = (bm1 == nullptr);
@@ -453,6 +463,16 @@ __global__ __launch_bounds__(WG) void k_mfma_autodiff(
       if (p1 == p0) break;
     }
   }
+}
+
+template <typename S, bool FUSED_D>
+__global__ __launch_bounds__(WG) void k_mfma_autodiff(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  mfma_autodiff_dev<S, FUSED_D>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out, dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
 }
 
 }  // namespace fkd

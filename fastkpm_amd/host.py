@@ -23,6 +23,7 @@ SIGNATURES = {
     "fkpm_host_set_R_identity": (_I, [_I, _I, _I, _I, _P]),
     "fkpm_host_set_R_uncorrelated": (_I, [_I, _I, _I, _P, _I, _I, _P]),
     "fkpm_host_set_R_correlated": (_I, [_I, _I, _IP, _P, _I, _I, _P, _IP]),
+    "fkpm_host_set_R_probing": (_I, [_I, _IP, _P, _IP, _P, _IP]),
     "fkpm_host_expansion_coefficients": (_I, [_I, _I, _I, _D, _D, _D, _D, _DP]),
     "fkpm_host_jackson_kernel": (None, [_I, _DP]),
     "fkpm_host_moment_product": (_D, [_I, _DP, _DP]),
@@ -98,6 +99,21 @@ def set_R_uncorrelated(dtype: str, n: int, s: int, rng: RNG, j0: int = 0,
     _check(lib().fkpm_host_set_R_uncorrelated(code, n, s, rng.ptr, j0, j1, R.ctypes.data_as(_P)),
            "set_R_uncorrelated")
     return R
+
+
+def set_R_probing(groups, rng: RNG):
+    """Compact form of the complex-double set_R_correlated(groups, rng): per
+    orbital row the column of its one nonzero (-1: none) and its value.
+    Returns (rcol int32[n], rval complex128[n], s)."""
+    g = np.ascontiguousarray(groups, dtype=np.int32)
+    n = len(g)
+    rcol = np.empty(n, dtype=np.int32)
+    rval = np.empty(n, dtype=np.complex128)
+    s = C.c_int()
+    _check(lib().fkpm_host_set_R_probing(n, g.ctypes.data_as(_IP), rng.ptr,
+                                         rcol.ctypes.data_as(_IP), rval.ctypes.data_as(_P),
+                                         C.byref(s)), "set_R_probing")
+    return rcol, rval, s.value
 
 
 def set_R_correlated(dtype: str, groups, rng: RNG, j0: int | None = None,

@@ -180,6 +180,64 @@ int fkpm_get_state(fkpm_handle* h, int* r_compact, int* tuned, int* moments_M);
  * counterpart in the reference. */
 int fkpm_bandwidth_probe(fkpm_handle* h, size_t bytes, int reps, double* gbs);
 
+/* ------------------------------------------------------------------------
+ * Batched engine (serving runtime).  No counterpart in the reference: it runs
+ * many independent Engine solves -- moments(M) then autodiff_matrix(c, D) on
+ * H's own pattern, the energy-and-force request of a simulation loop -- on one
+ * GPU at once.  A scheduler thread admits queued requests while a slot is free
+ * and the request's HBM footprint fits the batch's arena, advances every
+ * admitted request by one Chebyshev step per tick in batched launches
+ * (captured in hipGraphs), and retires finished ones.  Results do not depend on
+ * what else is in flight.  Complex-double inputs (the cx_double Engine types),
+ * block sizes 2 and 8, every diagonal block present, pattern structurally
+ * symmetric.
+ * ------------------------------------------------------------------------ */
+typedef struct fkpm_batch fkpm_batch;
+
+#define FKPM_BATCH_OPT_GRAPH 1 /* value 1 (default): capture runs of ticks in hipGraphs */
+#define FKPM_BATCH_OPT_TICKS 2 /* value: most ticks per replay between admissions (16) */
+
+/* HBM bytes one request occupies while admitted (its arena footprint). */
+int fkpm_batch_footprint(int prec, int nb, int b, int nnzb, int s, int M, size_t* bytes);
+
+/* Open a batched engine on `device` with an arena of `hbm_bytes` and at most
+ * `max_slots` requests in flight; prec = FKPM_PREC_FP64 or FKPM_PREC_FP32. */
+int fkpm_batch_open(int device, int prec, size_t hbm_bytes, int max_slots, fkpm_batch** out);
+int fkpm_batch_close(fkpm_batch* b);
+int fkpm_batch_set_option(fkpm_batch* b, int option, int value);
+
+/* Queue one request: H (BSR, nb block rows of b x b complex-double blocks,
+ * column-major), its energy scale [lo, hi], R (n x s complex double,
+ * column-major, n = nb * b), M (even) and the coefficients c[0..M-1] of the
+ * energy (expansion_coefficients of fermi_energy).  The inputs are copied.
+ * *id identifies the request. */
+int fkpm_batch_submit(fkpm_batch* b, int nb, int blk, int nnzb, const int* row_ptr,
+                      const int* col_idx, const void* val, double lo, double hi, int n, int s,
+                      const void* R_colmajor, int M, const double* c, long long* id);
+
+/* The same with probing vectors in compact form, one nonzero per orbital row
+ * (the R of Engine::set_R_correlated): rcol[i] in [-1, s) (-1: zero row) and
+ * rval[i] (complex double) for i < n. */
+int fkpm_batch_submit_probing(fkpm_batch* b, int nb, int blk, int nnzb, const int* row_ptr,
+                              const int* col_idx, const void* val, double lo, double hi, int n,
+                              int s, const int* rcol, const void* rval, int M, const double* c,
+                              long long* id);
+
+/* *done = 1 once the request has finished (or failed). */
+int fkpm_batch_poll(fkpm_batch* b, long long id, int* done);
+
+/* Wait for a request and take its results: mu[0..M-1] (moments), *energy =
+ * sum c_m mu_m, D_val (nnzb * b * b complex doubles, dE/dH on H's pattern as
+ * autodiff_matrix returns it; may be NULL) and times_ms[3] = queued, running,
+ * total milliseconds (may be NULL).  The request is forgotten afterwards. */
+int fkpm_batch_wait(fkpm_batch* b, long long id, double* mu, double* energy, double* D_val,
+                    double* times_ms);
+
+/* Ticks run, graph replays, graph captures, most requests in flight at once
+ * and the arena's high-water mark. */
+int fkpm_batch_stats(fkpm_batch* b, long long* ticks, long long* replays, long long* captures,
+                     int* peak_slots, size_t* peak_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,13 @@ int fkpm_host_set_R_uncorrelated(int dtype, int n, int s, void* rng, int j0, int
 int fkpm_host_set_R_correlated(int dtype, int n, const int* groups, void* rng, int j0, int j1,
                                void* R_out, int* s_out);
 
+/* The complex-double R of set_R_correlated(groups, rng) (two-argument form)
+ * in compact form, one nonzero per orbital row: rcol_out[i] = its column
+ * (-1 for none), rval_out[i] (complex double) = its value; the same draws as
+ * the dense form.  *s_out = column count. */
+int fkpm_host_set_R_probing(int n, const int* groups, void* rng, int* rcol_out, void* rval_out,
+                            int* s_out);
+
 /* expansion_coefficients(M, Mq, f, es) with f = fermi_energy(., kT, mu)
  * (kind 0), fermi_density(., kT, mu) (kind 1) or 1 (kind 2). */
 int fkpm_host_expansion_coefficients(int M, int Mq, int kind, double kT, double mu, double lo,

@@ -1,0 +1,155 @@
+"""Batched engine (serving runtime) through the C-ABI: each request against
+the CPU oracle (FP64 1e-10 relative, norm-wise), bitwise equality of a
+request's results alone and in a mix (graph-captured or eager ticks,
+admission-limited by an HBM budget), and the refusal paths."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import fermi_coeffs, rel_err, seeded_R
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def _req(oracle, kind, seed, M):
+    """(H, es, c, R) of a small request; R dense from the oracle's set_R_correlated."""
+    from fastkpm_amd import lattice, EnergyScale
+    if kind == "b2_2d":
+        shape, b, per = (6, 8), 2, (3, 4)
+        H = lattice.lattice(shape, b=2, J=1.5, W=2.0, seed=seed)
+    elif kind == "b2_3d":
+        shape, b, per = (4, 4, 4), 2, (2, 2, 2)
+        H = lattice.lattice(shape, b=2, J=1.0, W=4.0, seed=seed)
+    else:
+        shape, b, per = (3, 3, 2), 8, (3, 3, 2)
+        H = lattice.lattice(shape, b=8, orbital_blocks=True, seed=seed)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    es = EnergyScale(lam[0] - 0.2, lam[-1] + 0.2)
+    c = fermi_coeffs(oracle, M, es.lo, es.hi, kT=0.05)
+    groups = lattice.colouring(shape, per, b)
+    R = seeded_R(oracle, groups, seed + 100)
+    return H, es, c, R, groups
+
+
+def _oracle_solve(oracle, H, es, c, R):
+    orc = oracle.Engine()
+    orc.set_H(H, es.lo, es.hi)
+    orc.set_R(R)
+    mu = orc.moments(len(c))
+    D = orc.autodiff_matrix(c, H.like())
+    return mu, D
+
+
+MIX = [("b2_2d", 3, 40), ("b8", 5, 24), ("b2_3d", 7, 64), ("b2_2d", 9, 2), ("b8", 11, 36),
+       ("b2_3d", 13, 30)]
+
+
+@pytest.fixture(scope="module")
+def mix(oracle):
+    return [_req(oracle, k, s, M) for k, s, M in MIX]
+
+
+def test_batch_fp64_matches_oracle(oracle, mix):
+    from fastkpm_amd.serving import BatchEngine
+    eng = BatchEngine("fp64", hbm_bytes=1 << 30, max_slots=8)
+    ids = [eng.submit(H, es, c, R=R) for H, es, c, R, _ in mix]
+    for (H, es, c, R, _), rid in zip(mix, ids):
+        r = eng.wait(rid)
+        mu_o, D_o = _oracle_solve(oracle, H, es, c, R)
+        assert rel_err(r.mu, mu_o) < FP64_TOL
+        assert abs(r.energy - float(np.dot(c, mu_o))) <= FP64_TOL * abs(np.dot(c, mu_o)) + 1e-12
+        assert rel_err(r.D, D_o) < FP64_TOL
+    st = eng.stats()
+    assert st["captures"] >= 1 and st["ticks"] >= max(len(c) for _, _, c, _, _ in mix) + 2
+    eng.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_alone_equals_mix_bitwise(mix, prec):
+    """A request's results do not depend on what else is in flight, nor on
+    graph capture, nor on the tick run length."""
+    from fastkpm_amd.serving import BatchEngine
+    alone = []
+    eng = BatchEngine(prec, hbm_bytes=1 << 30, max_slots=8, graph=False)
+    for H, es, c, R, _ in mix:
+        alone.append(eng.wait(eng.submit(H, es, c, R=R)))
+    eng.close()
+    for graph, ticks in ((True, 16), (True, 5), (False, 3)):
+        eng = BatchEngine(prec, hbm_bytes=1 << 30, max_slots=4, graph=graph, max_ticks=ticks)
+        ids = [eng.submit(H, es, c, R=R) for H, es, c, R, _ in mix]
+        for a, rid in zip(alone, ids):
+            r = eng.wait(rid)
+            assert np.array_equal(r.mu, a.mu), (graph, ticks)
+            assert np.array_equal(r.D, a.D), (graph, ticks)
+            assert r.energy == a.energy
+        eng.close()
+
+
+def test_batch_fp32_mixed_against_oracle(oracle, mix):
+    from fastkpm_amd.serving import BatchEngine
+    eng = BatchEngine("fp32", hbm_bytes=1 << 30, max_slots=8)
+    ids = [eng.submit(H, es, c, R=R) for H, es, c, R, _ in mix]
+    for (H, es, c, R, _), rid in zip(mix, ids):
+        r = eng.wait(rid)
+        mu_o, D_o = _oracle_solve(oracle, H, es, c, R)
+        e_o = float(np.dot(c, mu_o))
+        assert abs(r.energy - e_o) / abs(e_o) < 1e-6
+        assert rel_err(r.D, D_o) < 1e-4
+    eng.close()
+
+
+def test_batch_admission_by_footprint(oracle, mix):
+    """With an arena that holds about two requests, all still complete, each
+    exactly as with room for all."""
+    from fastkpm_amd.serving import BatchEngine, footprint
+    fps = [footprint("fp64", H, R.shape[1], len(c)) for H, es, c, R, _ in mix]
+    budget = int(sorted(fps)[-1] + sorted(fps)[-2] + (1 << 20))
+    big = BatchEngine("fp64", hbm_bytes=1 << 30, max_slots=8)
+    ref = [big.wait(big.submit(H, es, c, R=R)) for H, es, c, R, _ in mix]
+    big.close()
+    eng = BatchEngine("fp64", hbm_bytes=budget, max_slots=8, max_ticks=7)
+    ids = [eng.submit(H, es, c, R=R) for H, es, c, R, _ in mix]
+    for a, rid in zip(ref, ids):
+        r = eng.wait(rid)
+        assert np.array_equal(r.mu, a.mu) and np.array_equal(r.D, a.D)
+    st = eng.stats()
+    assert st["peak_bytes"] <= budget
+    assert st["peak_slots"] < len(mix)
+    eng.close()
+
+
+def test_batch_probing_form_equals_dense(oracle):
+    """Compact probing vectors give the same results as the dense R they stand for."""
+    from fastkpm_amd import host
+    from fastkpm_amd.serving import BatchEngine
+    H, es, c, _, groups = _req(oracle, "b2_2d", 21, 48)
+    R = host.set_R_correlated("cx_double", groups, host.RNG(77))
+    probing = host.set_R_probing(groups, host.RNG(77))
+    eng = BatchEngine("fp64", hbm_bytes=1 << 28, max_slots=4)
+    a = eng.wait(eng.submit(H, es, c, R=R))
+    b = eng.wait(eng.submit(H, es, c, probing=probing))
+    assert np.array_equal(a.mu, b.mu) and np.array_equal(a.D, b.D)
+    eng.close()
+
+
+def test_batch_refusals(oracle):
+    from fastkpm_amd import lattice, EnergyScale
+    from fastkpm_amd._capi import FkpmError
+    from fastkpm_amd.serving import BatchEngine
+    eng = BatchEngine("fp64", hbm_bytes=1 << 20, max_slots=2)
+    H4 = lattice.lattice((4, 4), b=4, orbital_blocks=True, seed=1)
+    R4 = np.ones((H4.n_rows * 4, 2), dtype=np.complex128)
+    with pytest.raises(FkpmError, match="UNSUP"):
+        eng.submit(H4, EnergyScale(-10, 10), np.ones(8), R=R4)
+    Hb = lattice.lattice((16, 16, 16), b=2, seed=2)  # 8192 orbitals: 2 MiB per vector
+    Rb = np.ones((Hb.n_rows * 2, 16), dtype=np.complex128)
+    with pytest.raises(FkpmError, match="budget"):
+        eng.submit(Hb, EnergyScale(-12, 12), np.ones(64), R=Rb)
+    # a missing diagonal block
+    P = lattice.coo_to_bsr(2, 2, [0, 1], [1, 0], np.ones((2, 2, 2)))
+    with pytest.raises(FkpmError, match="diagonal"):
+        eng.submit(P, EnergyScale(-3, 3), np.ones(4), R=np.ones((4, 1), dtype=np.complex128))
+    eng.close()

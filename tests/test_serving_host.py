@@ -1,0 +1,47 @@
+"""Host side of the serving stream (no GPU): request generation, spectral
+bounds and compact probing vectors."""
+from __future__ import annotations
+
+import numpy as np
+
+from fastkpm_amd import host, lattice
+from fastkpm_amd.serving import build_request, draw_spec, gershgorin
+
+
+def test_gershgorin_bounds_contain_the_spectrum():
+    for H in (lattice.lattice((6, 6), b=2, J=2.0, W=3.0, seed=4),
+              lattice.lattice((3, 3, 2), b=8, orbital_blocks=True, seed=5)):
+        lam = np.linalg.eigvalsh(H.to_dense())
+        es = gershgorin(H)
+        assert es.lo < lam[0] and lam[-1] < es.hi
+
+
+def test_probing_form_is_the_dense_R():
+    for shape, per, b in (((8, 8), (4, 4), 2), ((4, 4, 2), (2, 2, 2), 8)):
+        g = lattice.colouring(shape, per, b)
+        R = host.set_R_correlated("cx_double", g, host.RNG(19))
+        rcol, rval, s = host.set_R_probing(g, host.RNG(19))
+        D = np.zeros_like(R)
+        D[np.arange(len(g)), rcol] = rval
+        assert s == R.shape[1] and np.array_equal(D, R)
+    # colour count reaching n: identity, no draws
+    rcol, rval, s = host.set_R_probing(np.arange(6), host.RNG(1))
+    assert s == 6 and np.array_equal(rcol, np.arange(6)) and np.all(rval == 1)
+
+
+def test_stream_specs_stay_in_range():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        sp = draw_spec(rng)
+        assert 4096 <= sp.N <= 1 << 20
+        assert 256 <= sp.M <= 2048 and sp.M % 2 == 0
+        assert 16 <= sp.S <= 128
+        assert 0.0 <= sp.W <= 6.0
+
+
+def test_build_request_small():
+    rng = np.random.default_rng(3)
+    sp = draw_spec(rng, n_max=5000, m_max=256)
+    H, es, c, (rcol, rval, s) = build_request(sp)
+    assert H.n_rows * H.b_len == sp.N and len(c) == sp.M and s == sp.S
+    assert es.lo < es.hi and len(rcol) == sp.N
