@@ -14,6 +14,10 @@ colouring x 2 orbitals), Fermi function at mu = 0, kT = 0.01, FP64.
 
 Prints one JSON line.  The CPU baseline times the FP64 oracle (single thread)
 on the same workload with M' moments (a bounded sample) and reports N*M'*S/s.
+The "serving" entry runs a seeded Poisson stream of mixed requests (energy +
+dE/dH each) through the batched engine, FP64 and FP32-mixed, and reports
+p50/p99 latency and aggregate updates/s; --serve-only runs that stream alone,
+at the full request ranges, and prints it as its own JSON line.
 """
 from __future__ import annotations
 
@@ -30,6 +34,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 from fastkpm_amd import HipEngine, bandwidth_probe, device_count, host, lattice  # noqa: E402
+from fastkpm_amd import serving  # noqa: E402
+from fastkpm_amd.engine import EnergyScale  # noqa: E402
 
 METRIC = ("N·M·S Chebyshev updates/s, FP32-mixed vs FP64 at ≤1e-6 energy err; "
           "p99 lat")
@@ -38,6 +44,18 @@ HBM_PEAK_GBS = 8000.0
 
 def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+WORKLOADS = {
+    "C1": "square lattice 32x32 periodic, t=-1, real H, N=1024, es=[-4.1,4.1], M=500, "
+          "S=16 probing colours (4x4), fermi mu=0 kT=0",
+    "C2": "Kondo lattice 256x256 sites x 2 spin orbitals, complex 2x2 BSR, t=-1, J=2, "
+          "M=1000, S=64 (4x8 colours x 2 spins), kT=0.01",
+    "C3": "cubic 64^3 sites x 2 spin orbitals, Anderson W=4 + Kondo J=1, complex 2x2 BSR, "
+          "M=2000, S=128 (4x4x4 colours x 2), kT=0.01",
+    "C4": "cubic 32^3 sites with 8x8 orbital blocks (random Hermitian on-site, complex "
+          "Gaussian(0,0.5) hopping), M=1024, S=256 (4x4x2 colours x 8), kT=0.01",
+}
 
 
 def build_workload(name: str, seed: int):
@@ -84,7 +102,7 @@ def measured_traffic(config: str, prec: str, kernel: str):
 def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     M = cfg["M"]
     eng = HipEngine("cx_double", prec)
-    es = eng.energy_scale(H, 0.01)
+    es = EnergyScale(*cfg["es"]) if "es" in cfg else eng.energy_scale(H, 0.01)
     eng.set_R_correlated(groups, host.RNG(seed))
     eng.set_H(H, es)
     t_tune = time.perf_counter()
@@ -141,6 +159,98 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
                 kernel_config=kcfg, tune_s=t_tune, state=eng.state())
 
 
+SMALL_SHAPES = {"C1": (32, 32), "C2": (16, 16), "C3": (8, 8, 8), "C4": (4, 4, 4)}
+
+
+def _fermi_energy(x, kT, mu):
+    """fermi_energy of src/fastkpm.cpp:375-383, elementwise."""
+    x = np.asarray(x, dtype=float)
+    if kT < 1e-15:
+        return np.where(x < mu, x - mu, 0.0)
+    a = (x - mu) / abs(kT)
+    soft = -kT * np.log1p(np.exp(-np.clip(a, -20, 20)))
+    return np.where(np.abs(a) > 20, np.where(x < mu, x - mu, 0.0), soft)
+
+
+def _fermi_density(x, kT, mu):
+    """fermi_density of src/fastkpm.cpp:385-393, elementwise."""
+    x = np.asarray(x, dtype=float)
+    if kT < 1e-15:
+        return np.where(x < mu, 1.0, 0.0)
+    a = (x - mu) / abs(kT)
+    soft = 1.0 / (np.exp(np.clip(a, -20, 20)) + 1.0)
+    return np.where(np.abs(a) > 20, np.where(x < mu, 1.0, 0.0), soft)
+
+
+def exact_check(name: str, prec: str, seed: int) -> dict:
+    """KPM on the GPU against exact FP64 diagonalisation of f(H) on a small
+    lattice of the workload's family, R = identity (no stochastic error): the
+    remaining difference is the Chebyshev truncation (Jackson kernel, M)."""
+    cfg = lattice.CONFIGS[name]
+    H, _, _ = lattice.make_config(name, seed=seed, scale_shape=SMALL_SHAPES[name])
+    n = H.n_rows * H.b_len
+    M = cfg["M"]
+    eng = HipEngine("cx_double", prec)
+    es = EnergyScale(*cfg["es"]) if "es" in cfg else eng.energy_scale(H, 0.01)
+    eng.set_R(np.eye(n, dtype=np.complex128))
+    eng.set_H(H, es)
+    cE = host.expansion_coefficients(M, 2 * M, host.FERMI_ENERGY, cfg["kT"], cfg["mu"], es.lo,
+                                     es.hi)
+    cF = host.expansion_coefficients(M, 2 * M, host.FERMI_DENSITY, cfg["kT"], cfg["mu"], es.lo,
+                                     es.hi)
+    e_kpm = float(np.dot(cE, eng.moments(M)))
+    Dg, Ds = H.like(), H.like()
+    eng.autodiff_matrix(cE, Dg)
+    eng.stoch_matrix(cF, Ds)
+    eng.close()
+    lam, V = np.linalg.eigh(H.to_dense())
+    e_ex = float(np.sum(_fermi_energy(lam, cfg["kT"], cfg["mu"])))
+    F = (V * _fermi_density(lam, cfg["kT"], cfg["mu"])) @ V.conj().T  # = dE/dH = f(H)
+    mask = np.abs(H.to_dense()) > 0
+    fm = F[mask]
+    return {"lattice": "x".join(map(str, SMALL_SHAPES[name])) + f" x {H.b_len} orbitals",
+            "N": n, "M": M, "precision": prec,
+            "energy_rel_err": abs(e_kpm - e_ex) / abs(e_ex),
+            "dEdH_rel_err": float(np.max(np.abs(Dg.to_dense()[mask] - fm)) / np.max(np.abs(fm))),
+            "Dij_rel_err": float(np.max(np.abs(Ds.to_dense()[mask] - fm)) / np.max(np.abs(fm)))}
+
+
+def cpu_baseline_threads(H, groups, cfg, es, seed: int, m_sample: int, threads: int):
+    """The oracle on S-column shards, one thread per shard (the split of the
+    reference's Engine_Threaded): moments, D summed over shards."""
+    import concurrent.futures as cf
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py
+    R = host.set_R_correlated("cx_double", groups, host.RNG(seed))
+    S = R.shape[1]
+    N = H.n_rows * H.b_len
+    cE = host.expansion_coefficients(m_sample, 2 * m_sample, host.FERMI_ENERGY, cfg["kT"],
+                                     cfg["mu"], es.lo, es.hi)
+    cF = host.expansion_coefficients(m_sample, 2 * m_sample, host.FERMI_DENSITY, cfg["kT"],
+                                     cfg["mu"], es.lo, es.hi)
+    cuts = np.linspace(0, S, threads + 1).astype(int)
+    shards = [R[:, a:b] for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+
+    def work(Rs):
+        orc = oracle_py.Engine()
+        orc.set_H(H, es.lo, es.hi)
+        orc.set_R(np.asfortranarray(Rs))
+        D = H.like()
+        orc.moments(m_sample)
+        orc.autodiff_matrix(cE, D)
+        orc.stoch_matrix(cF, D)
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(len(shards)) as ex:
+        list(ex.map(work, shards))
+    dt = time.perf_counter() - t0
+    return {"value": N * m_sample * S / dt, "unit": "updates/s", "cores": len(shards),
+            "kind": "port",
+            "sample": (f"same workload and M'={m_sample} as the one-thread figure, R split into "
+                       f"{len(shards)} column shards run concurrently (oracle calls release the "
+                       f"GIL) in {dt:.2f} s")}
+
+
 def cpu_baseline(H, groups, cfg, es, seed: int, m_sample: int):
     """The FP64 oracle (the reference CPU algorithm restated in C, one thread)
     on the same H, R and coefficients with M' = m_sample moments."""
@@ -169,6 +279,43 @@ def cpu_baseline(H, groups, cfg, es, seed: int, m_sample: int):
                        f"{os.cpu_count()} cores")}
 
 
+def serving_leg(n_req: int, n_max: int, m_max: int, rate: float, seed: int, hbm_gb: int,
+                precs=("fp64", "fp32")) -> dict:
+    """The mixed Poisson request stream through the batched engine."""
+    rng = np.random.default_rng(seed)
+    specs = [serving.draw_spec(rng, n_max=n_max, m_max=m_max) for _ in range(n_req)]
+    out = {"stream": (f"{n_req} requests, Poisson arrivals at {rate}/s (seed {seed}), then the "
+                      f"same requests queued at once ('saturated'); 2-D/3-D "
+                      f"lattices, 2-spin or 8-orbital blocks, N 4096..{n_max}, M 256..{m_max}, "
+                      f"S 16..128, W in [0,6]; each request = moments + autodiff_matrix "
+                      f"(energy and dE/dH), HBM arena {hbm_gb} GiB"),
+           "updates_total": int(sum(sp.N * sp.M * sp.S for sp in specs))}
+    for prec in precs:
+        eng = serving.BatchEngine(prec, hbm_bytes=hbm_gb << 30, max_slots=64)
+        # first use: kernel loading and the first graph capture
+        warm = serving.draw_spec(np.random.default_rng(seed + 1), n_max=8192, m_max=256)
+        H, es, c, pr = serving.build_request(warm)
+        eng.wait(eng.submit(H, es, c, probing=pr), want_D=False)
+        recs, wall = serving.run_stream(eng, specs, rate, seed=seed)
+        summ = serving.summarize(recs, wall)
+        summ["energies"] = [r.energy for _, r in recs]
+        # capacity: the same requests all queued at once
+        recs2, wall2 = serving.run_stream(eng, specs, 1e9, seed=seed)
+        sat = serving.summarize(recs2, wall2)
+        summ["saturated"] = {"updates_per_s": sat["updates_per_s"], "wall_s": sat["wall_s"],
+                             "p50_ms": sat["p50_ms"], "p99_ms": sat["p99_ms"]}
+        summ.update(eng.stats())
+        out[prec] = summ
+        eng.close()
+    if "fp64" in out and "fp32" in out:
+        e64 = np.array(out["fp64"].pop("energies"))
+        e32 = np.array(out["fp32"].pop("energies"))
+        out["fp32_energy_rel_err_max"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+    for prec in precs:
+        out[prec].pop("energies", None)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +327,12 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-sample-moments", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-mixed comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving stream, at the full request ranges")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--serve-hbm-gb", type=int, default=64)
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -191,6 +344,17 @@ def main(argv=None) -> int:
     if device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+
+    if args.serve_only:
+        _log("serving stream (full ranges)")
+        sv = serving_leg(args.serve_requests, 1 << 20, 2048, args.serve_rate, args.seed,
+                         args.serve_hbm_gb)
+        print(json.dumps({"metric": "batched serving: aggregate N·M·S updates/s and p50/p99 "
+                                    "latency of a mixed KPM request stream",
+                          "value": sv["fp64"]["updates_per_s"], "unit": "updates/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic", "serving": sv}), flush=True)
+        return 0
 
     # the practical streaming ceiling on this device: a plain 1 GiB copy
     # (k_copy16); its dispatches also calibrate the HBM byte counters of a
@@ -218,16 +382,32 @@ def main(argv=None) -> int:
                    "frac": round(gbs / HBM_PEAK_GBS, 4),
                    "ms_per_step": round(ms / args.steps, 3)}
     dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
-    cfgk = r["config"]
-    names = {"moments": "k_step_moments", "stoch": "k_step_moments (Clenshaw)",
-             "autodiff": "k_step_autodiff"}
     kc = r["kernel_config"]["autodiff" if dom == "autodiff" else "moments"]
+    if kc.get("mfma"):
+        names = {"moments": "k_mfma_step", "stoch": "k_mfma_step (Clenshaw)",
+                 "autodiff": "k_mfma_autodiff"}
+        geo = f"b={H.b_len}, matrix cores, wg={kc['n_wg']}"
+    else:
+        names = {"moments": "k_step_moments", "stoch": "k_step_moments (Clenshaw)",
+                 "autodiff": "k_step_autodiff"}
+        geo = f"b={H.b_len}, lanes={kc['lanes']}, kq={kc['kq']}, wg={kc['n_wg']}, nt={kc['nt']}"
     roof = {"bound": "hbm", "achieved": kern[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None,
-            "kernel": (f"{names[dom]} (b={H.b_len}, lanes={kc['lanes']}, kq={kc['kq']}, "
-                       f"wg={kc['n_wg']}, nt={kc['nt']})"),
+            "kernel": f"{names[dom]} ({geo})",
             "bytes_per_launch": kern[dom]["bytes_per_launch"],
             "avg_launch_us": kern[dom]["avg_launch_us"]}
+    # arithmetic of the block products: 8 b^2 real flops per block and column
+    # (one SpMM in moments / stoch, two SpMMs and the outer product in autodiff)
+    fl = {"moments": 1, "stoch": 1, "autodiff": 3}
+    for k in kern:
+        flops = fl[k] * H.n_blocks() * S * 8 * H.b_len ** 2
+        kern[k]["flops_per_launch"] = flops
+        kern[k]["achieved_TFLOPs"] = round(flops / (kern[k]["avg_launch_us"] * 1e-6) / 1e12, 2)
+    if kc.get("mfma"):
+        peak_tf = 78.6 if args.precision == "fp64" else 157.3
+        roof["mfma"] = {"achieved": kern[dom]["achieved_TFLOPs"], "peak": peak_tf,
+                        "unit": "TFLOP/s",
+                        "frac": round(kern[dom]["achieved_TFLOPs"] / peak_tf, 4)}
     roof["copy_GBs"] = round(copy_gbs, 1)
     roof["frac_of_copy"] = round(kern[dom]["achieved_GBs"] / copy_gbs, 4)
     tr = measured_traffic(args.config, args.precision, names[dom])
@@ -240,9 +420,8 @@ def main(argv=None) -> int:
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32", "data": "synthetic",
-        "config": {"workload": (f"{args.config}: Kondo lattice 256x256x2 orbitals, complex 2x2 BSR, "
-                                f"J=2, M={M}, S={S}, kT=0.01; step = moments + autodiff_matrix "
-                                f"+ stoch_matrix"),
+        "config": {"workload": (f"{args.config}: {WORKLOADS.get(args.config, args.config)}; "
+                                f"step = moments + autodiff_matrix + stoch_matrix"),
                    "N": N, "M": M, "S": S, "b": H.b_len, "nnzb": H.n_blocks(),
                    "precision": args.precision, "engine_T": "cx_double"},
         "latency_ms": {"p50": round(float(np.percentile(times_ms, 50)), 3),
@@ -265,10 +444,21 @@ def main(argv=None) -> int:
                              "energy_rel_err_vs_fp64": e_err,
                              "dEdH_rel_err_vs_fp64": dg_err, "Dij_rel_err_vs_fp64": ds_err}
         del r32
+    if not args.no_serve:
+        _log("serving stream")
+        out["serving"] = serving_leg(args.serve_requests, 1 << 18, 1024, args.serve_rate,
+                                     args.seed, args.serve_hbm_gb)
+    _log("exact-diagonalisation check")
+    out["exact_check"] = exact_check(args.config, args.precision, args.seed)
     if not args.no_cpu_baseline:
         _log(f"cpu baseline (oracle, M'={args.cpu_sample_moments})")
         out["cpu_baseline"] = cpu_baseline(H, groups, cfg, r["es"], args.seed,
                                            args.cpu_sample_moments)
+        threads = min(16, os.cpu_count() or 1, S)
+        if threads > 1:
+            _log(f"cpu baseline, {threads} column shards")
+            out["cpu_baseline_threads"] = cpu_baseline_threads(
+                H, groups, cfg, r["es"], args.seed, args.cpu_sample_moments, threads)
     print(json.dumps(out), flush=True)
     return 0
 
