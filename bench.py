@@ -384,8 +384,9 @@ def main(argv=None) -> int:
     dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
     kc = r["kernel_config"]["autodiff" if dom == "autodiff" else "moments"]
     if kc.get("mfma"):
-        names = {"moments": "k_mfma_step", "stoch": "k_mfma_step (Clenshaw)",
-                 "autodiff": "k_mfma_autodiff"}
+        rr = "_rr" if kc.get("mfma_variant") == 2 else ""
+        names = {"moments": f"k_mfma_step{rr}", "stoch": f"k_mfma_step{rr} (Clenshaw)",
+                 "autodiff": f"k_mfma_autodiff{rr}"}
         geo = f"b={H.b_len}, matrix cores, wg={kc['n_wg']}"
     else:
         names = {"moments": "k_step_moments", "stoch": "k_step_moments (Clenshaw)",

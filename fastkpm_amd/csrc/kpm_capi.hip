@@ -147,7 +147,8 @@ enum KKind { K_MOM = 0, K_XI = 1, K_AD = 2, K_OUT = 3, NKIND = 4 };
 // workgroup cap and nontemporal bits; nwg/chunk derived from them.
 struct KCfg {
   int L = 0, KQ = 0, maxwg = 0, nt = 0;
-  int mfma = 0;  // matrix-core kernel (one wave per block row: L = 64, KQ = 1)
+  int mfma = 0;  // matrix-core kernel (one wave per block row: L = 64, KQ = 1):
+                 // 1 general, 2 row-resident (rows of <= 8 blocks, vectors < 4 GiB)
   int nwg = 0, chunk = 0;
 };
 
@@ -172,7 +173,7 @@ struct fkpm_handle {
   bool single = false;  // storage is FP32
 
   // --- H (scaled) ---
-  int nb = 0, b = 0, nnzb = 0;
+  int nb = 0, b = 0, nnzb = 0, max_deg = 0;
   bool h_set = false;
   DevBuf rp, ci, hv, rowblk, hraw;
   double avg = 0.0, mag = 1.0;
@@ -272,9 +273,15 @@ bool mfma_avail(const fkpm_handle* h, int kind) {
   return h->mfma_on && h->cx && h->b == 8 && (kind == K_MOM || kind == K_AD);
 }
 
+// The row-resident variants: rows of at most 8 blocks, byte offsets of a
+// vector within 32 bits.
+bool mfma_rr_avail(const fkpm_handle* h, int kind) {
+  return mfma_avail(h, kind) && h->max_deg <= 8 && h->vbytes() < (size_t(1) << 32);
+}
+
 int set_kind(fkpm_handle* h, int kind, KCfg c) {
   if (c.mfma) {
-    if (!mfma_avail(h, kind))
+    if (!mfma_avail(h, kind) || (c.mfma == 2 && !mfma_rr_avail(h, kind)))
       return fail(FKPM_E_UNSUP, "no matrix-core kernel for block %d (kind %d)", h->b, kind);
     c.L = 64;  // one wave per block row
     c.KQ = 1;
@@ -290,11 +297,11 @@ int set_kind(fkpm_handle* h, int kind, KCfg c) {
 // Kernel entry of a kind under its configuration.
 template <class S>
 auto mom_fn(const fkd::KSet<S>& ks, const KCfg& c) {
-  return c.mfma ? ks.mfma_mom : ks.mom;
+  return c.mfma == 2 ? ks.mfma_mom_rr : c.mfma ? ks.mfma_mom : ks.mom;
 }
 template <class S>
 auto adf_fn(const fkd::KSet<S>& ks, const KCfg& c) {
-  return c.mfma ? ks.mfma_ad_fused : ks.ad_fused;
+  return c.mfma == 2 ? ks.mfma_ad_rr : c.mfma ? ks.mfma_ad_fused : ks.ad_fused;
 }
 template <class S>
 auto adp_fn(const fkd::KSet<S>& ks, const KCfg& c) {
@@ -329,7 +336,11 @@ KCfg default_cfg(fkpm_handle* h, int kind) {
   }
   c.maxwg = 4096;
   c.nt = 0;
-  if (mfma_avail(h, kind)) c.mfma = 1;
+  if (mfma_avail(h, kind)) {
+    c.mfma = 1;
+    // row-resident: the forward step always, the autodiff step in FP64
+    if (mfma_rr_avail(h, kind) && (kind == K_MOM || !h->single)) c.mfma = 2;
+  }
   return c;
 }
 
@@ -339,8 +350,8 @@ std::map<std::string, std::array<KCfg, NKIND>> g_tune_cache;
 
 std::string tune_key(const fkpm_handle* h) {
   char k[128];
-  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb, h->nnzb,
-           h->mfma_on ? 1 : 0);
+  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb,
+           h->nnzb, h->mfma_on ? 1 : 0, h->max_deg <= 8 ? 1 : 0);
   return k;
 }
 
@@ -555,12 +566,13 @@ int tune_t(fkpm_handle* h) {
         }
     }
     if (mfma_avail(h, kind))
-      for (int cap : caps) {
-        KCfg c;
-        c.mfma = 1;
-        c.maxwg = cap;
-        cands.push_back(c);
-      }
+      for (int v = 1; v <= (mfma_rr_avail(h, kind) ? 2 : 1); v++)
+        for (int cap : caps) {
+          KCfg c;
+          c.mfma = v;
+          c.maxwg = cap;
+          cands.push_back(c);
+        }
     {
       for (const KCfg& c : cands) {
           CHK(set_kind(h, kind, c));
@@ -885,6 +897,8 @@ int upload_pattern(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr,
   h->b = b;
   h->nnzb = nnzb;
   h->h_rp.assign(row_ptr, row_ptr + nb + 1);
+  h->max_deg = 0;
+  for (int i = 0; i < nb; i++) h->max_deg = std::max(h->max_deg, row_ptr[i + 1] - row_ptr[i]);
   h->h_ci.assign(col_idx, col_idx + nnzb);
   std::vector<int> rowblk(nnzb > 0 ? nnzb : 1);
   for (int i = 0; i < nb; i++)

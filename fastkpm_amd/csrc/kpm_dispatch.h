@@ -40,6 +40,13 @@ struct KSet {
   void (*mfma_ad_plain)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
                         S*, S*, double*, int, int, int, double, double, double, double, double,
                         double, int) = nullptr;
+  // row-resident matrix-core variants (rows of at most 8 blocks, vectors
+  // under 4 GiB): forward step, fused autodiff step
+  void (*mfma_mom_rr)(const int*, const int*, const S*, const S*, const S*, S*, int, int, int,
+                      double, double, double*, int, RSource<S>) = nullptr;
+  void (*mfma_ad_rr)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
+                     S*, S*, double*, int, int, int, double, double, double, double, double,
+                     double, int) = nullptr;
   int B = 0, L = 0, KQ = 0;
 };
 

@@ -78,17 +78,6 @@ __device__ __forceinline__ MfmaLane mfma_lane() {
   return m;
 }
 
-// The four A-operand values (K chunks 0..3: re/im of orbital q, re/im of
-// orbital q + 4) of one Hs block for this lane.
-template <typename S>
-__device__ __forceinline__ void mfma_a(const S* __restrict__ blk, const MfmaLane& m, S (&a)[4]) {
-  const S hr0 = blk[m.h0], hi0 = blk[m.h0 + 1];
-  const S hr1 = blk[m.h1], hi1 = blk[m.h1 + 1];
-  a[0] = m.im ? hi0 : hr0;
-  a[1] = m.im ? hr0 : -hi0;
-  a[2] = m.im ? hi1 : hr1;
-  a[3] = m.im ? hr1 : -hi1;
-}
 
 template <typename S>
 __device__ __forceinline__ typename Mfma<S>::v4 mfma_block(const S (&a)[4],
@@ -112,53 +101,65 @@ __device__ __forceinline__ void stc(S* __restrict__ p, typename Mfma<S>::c2 v) {
   *reinterpret_cast<typename Mfma<S>::c2*>(p) = v;
 }
 
-// Neighbour blocks gathered per batch and 16-column tiles per pass.
+// This lane's raw values of one Hs block: Hs(O, q) and Hs(O, q + 4).
+template <typename S>
+struct HRaw {
+  typename Mfma<S>::c2 e0, e1;
+};
+
+template <typename S>
+__device__ __forceinline__ HRaw<S> mfma_hraw(const S* __restrict__ blk, const MfmaLane& m) {
+  return HRaw<S>{ldc<S>(blk + m.h0), ldc<S>(blk + m.h1)};
+}
+
+// The four A-operand values (K chunks 0..3: re/im of orbital q, re/im of
+// orbital q + 4) of a block, times w (0 for a padding slot of a batch).  Kept
+// apart from the loads so that a batch issues all its loads before any use.
+template <typename S>
+__device__ __forceinline__ void mfma_a_of(const HRaw<S>& h, const MfmaLane& m, S w, S (&a)[4]) {
+  a[0] = w * (m.im ? h.e0.y : h.e0.x);
+  a[1] = w * (m.im ? h.e0.x : -h.e0.y);
+  a[2] = w * (m.im ? h.e1.y : h.e1.x);
+  a[3] = w * (m.im ? h.e1.x : -h.e1.y);
+}
+
+// Neighbour blocks gathered per batch (MF_U) and 16-column tiles per pass
+// (MF_TU) of the forward step.
 constexpr int MF_U = 4;
 constexpr int MF_TU = 2;
 
-// Y tiles t0, t0 + 1 of block row i: acc[tt] += sum_p Hs_p X_{j_p} over the
-// row's blocks [p0, p1), X gathered at column offset (scalars) co[tt].
-template <typename S>
+// Y tiles t0 .. t0 + TU - 1 of block row i: acc[tt] += sum_p Hs_p X_{j_p}
+// over the row's blocks [p0, p1) (p1 > p0), X gathered at column offset
+// (scalars) co[tt], which the caller keeps inside the row.  The loads of a
+// batch have no branches between them: a padding slot past the row's end
+// reloads the row's last block and enters the products with weight 0.
+template <typename S, int MF_U = fkd::MF_U, int MF_TU = fkd::MF_TU>
 __device__ __forceinline__ void mfma_gather(const int* __restrict__ ci, const S* __restrict__ hv,
                                             const S* __restrict__ X, size_t rs, const MfmaLane& m,
                                             int p0, int p1, const size_t (&co)[MF_TU],
-                                            const bool (&tv)[MF_TU],
                                             typename Mfma<S>::v4 (&acc)[MF_TU]) {
   using C2 = typename Mfma<S>::c2;
   for (int p = p0; p < p1; p += MF_U) {
-    S a[MF_U][4];
+    HRaw<S> hr[MF_U];
     C2 x0[MF_U][MF_TU], x1[MF_U][MF_TU];
 #pragma unroll
     for (int u = 0; u < MF_U; u++) {
-      if (p + u < p1) {
-        const int j = ci[p + u];
-        mfma_a<S>(hv + (size_t)(p + u) * 128, m, a[u]);
-        const size_t r0 = (size_t)(8 * j + m.q) * rs, r1 = (size_t)(8 * j + 4 + m.q) * rs;
+      const int pu = min(p + u, p1 - 1);
+      const int j = ci[pu];
+      hr[u] = mfma_hraw<S>(hv + (size_t)pu * 128, m);
+      const size_t r0 = (size_t)(8 * j + m.q) * rs, r1 = (size_t)(8 * j + 4 + m.q) * rs;
 #pragma unroll
-        for (int tt = 0; tt < MF_TU; tt++) {
-          if (tv[tt]) {
-            x0[u][tt] = ldc<S>(X + r0 + co[tt]);
-            x1[u][tt] = ldc<S>(X + r1 + co[tt]);
-          } else {
-            x0[u][tt] = C2{S(0), S(0)};
-            x1[u][tt] = C2{S(0), S(0)};
-          }
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) a[u][k] = S(0);
-#pragma unroll
-        for (int tt = 0; tt < MF_TU; tt++) {
-          x0[u][tt] = C2{S(0), S(0)};
-          x1[u][tt] = C2{S(0), S(0)};
-        }
+      for (int tt = 0; tt < MF_TU; tt++) {
+        x0[u][tt] = ldc<S>(X + r0 + co[tt]);
+        x1[u][tt] = ldc<S>(X + r1 + co[tt]);
       }
     }
 #pragma unroll
     for (int u = 0; u < MF_U; u++) {
-      if (p + u >= p1) continue;
+      S a[4];
+      mfma_a_of<S>(hr[u], m, (p + u < p1) ? S(1) : S(0), a);
 #pragma unroll
-      for (int tt = 0; tt < MF_TU; tt++) acc[tt] = mfma_block<S>(a[u], x0[u][tt], x1[u][tt], acc[tt]);
+      for (int tt = 0; tt < MF_TU; tt++) acc[tt] = mfma_block<S>(a, x0[u][tt], x1[u][tt], acc[tt]);
     }
   }
 }
@@ -183,7 +184,7 @@ __device__ __forceinline__ void mfma_rsource(const RSource<S>& rs_, size_t row, 
 //   y = cx (Hs x) + cz z [+ cr R], dots sum |x_i|^2 and Re<y_i, x_i>.
 // Same contract as k_step_moments (engine_cpu.cpp:63-84 fused with both cdot's).
 // ---------------------------------------------------------------------------
-template <typename S>
+template <typename S, int MF_U = fkd::MF_U, int MF_TU = fkd::MF_TU>
 __device__ __forceinline__ void mfma_step_dev(
     WgId wid,
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
@@ -212,11 +213,12 @@ __device__ __forceinline__ void mfma_step_dev(
       V4 acc[MF_TU];
 #pragma unroll
       for (int tt = 0; tt < MF_TU; tt++) {
-        co[tt] = (size_t)(t0 + tt) * 32 + 2 * m.c;
+        // a tile past the row's end recomputes the last one; its result is dropped
+        co[tt] = (size_t)min(t0 + tt, ntile - 1) * 32 + 2 * m.c;
         tv[tt] = (t0 + tt) < ntile;
         acc[tt] = V4{S(0), S(0), S(0), S(0)};
       }
-      mfma_gather<S>(ci, hv, x, rs, m, p0, p1, co, tv, acc);
+      if (p1 > p0) mfma_gather<S, MF_U, MF_TU>(ci, hv, x, rs, m, p0, p1, co, acc);
 #pragma unroll
       for (int tt = 0; tt < MF_TU; tt++) {
         if (!tv[tt]) continue;
@@ -255,6 +257,425 @@ __global__ __launch_bounds__(WG) void k_mfma_step(
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
     int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
   mfma_step_dev<S>(wg_self(), rp, ci, hv, x, z, y, nb, qp, chunk, cx, cz, partial, nt, rsrc);
+}
+
+// ---------------------------------------------------------------------------
+// Row-resident forward step for rows of at most DM blocks.  A block row's A
+// operands (selected and signed Hs values) and its neighbours' row offsets
+// are computed once per row; the tile loop then issues only buffer loads
+// (SGPR-based resource, 32-bit per-lane offsets, the tile's offset in an
+// SGPR), MFMAs and the epilogue.  Same contract as mfma_step_dev; the host
+// uses it when every row has at most DM blocks and a vector spans less than
+// 4 GiB.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vec_rsrc(const void* p, uint32_t bytes) {
+  // raw buffer (stride 0, dword 3 of the gfx9 resource); reads past `bytes` return 0
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// Raw buffer loads / stores of one complex element, bound to the LLVM
+// intrinsics (one dwordx2 / dwordx4 access; voffset per lane, soffset uniform).
+typedef float bf32x2 __attribute__((ext_vector_type(2)));
+typedef double bf64x2 __attribute__((ext_vector_type(2)));
+__device__ bf32x2 fkd_raw_ld_f32x2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+__device__ bf64x2 fkd_raw_ld_f64x2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f64");
+__device__ void fkd_raw_st_f32x2(bf32x2 v, __amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                 int aux) __asm("llvm.amdgcn.raw.ptr.buffer.store.v2f32");
+__device__ void fkd_raw_st_f64x2(bf64x2 v, __amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                 int aux) __asm("llvm.amdgcn.raw.ptr.buffer.store.v2f64");
+
+__device__ float fkd_raw_ld_f32(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                               int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f32");
+__device__ double fkd_raw_ld_f64(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f64");
+
+template <typename S>
+struct BufC2;
+template <>
+struct BufC2<float> {
+  static __device__ __forceinline__ float2 ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    const bf32x2 v = fkd_raw_ld_f32x2(r, (int)vo, (int)so, 0);
+    return float2{v.x, v.y};
+  }
+  static __device__ __forceinline__ void st(float2 d, __amdgpu_buffer_rsrc_t r, uint32_t vo,
+                                            uint32_t so) {
+    fkd_raw_st_f32x2(bf32x2{d.x, d.y}, r, (int)vo, (int)so, 0);
+  }
+  static __device__ __forceinline__ float ld1(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    return fkd_raw_ld_f32(r, (int)vo, (int)so, 0);
+  }
+};
+template <>
+struct BufC2<double> {
+  static __device__ __forceinline__ double2 ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    const bf64x2 v = fkd_raw_ld_f64x2(r, (int)vo, (int)so, 0);
+    return double2{v.x, v.y};
+  }
+  static __device__ __forceinline__ void st(double2 d, __amdgpu_buffer_rsrc_t r, uint32_t vo,
+                                            uint32_t so) {
+    fkd_raw_st_f64x2(bf64x2{d.x, d.y}, r, (int)vo, (int)so, 0);
+  }
+  static __device__ __forceinline__ double ld1(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    return fkd_raw_ld_f64(r, (int)vo, (int)so, 0);
+  }
+};
+
+constexpr int MF_DM = 8;  // most blocks per row of the row-resident kernels
+
+template <typename S, int DM = MF_DM, int G = 4>
+__device__ __forceinline__ void mfma_step_rr_dev(
+    WgId wid, const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
+  using C2 = typename Mfma<S>::c2;
+  using V4 = typename Mfma<S>::v4;
+  using IO = BufC2<S>;
+  constexpr int NS = Vec16<S>::N;
+  constexpr uint32_t ES = sizeof(S);
+  (void)nt;
+  const MfmaLane m = mfma_lane<S>();
+  const int g = threadIdx.x / WAVE;
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
+  const uint32_t rs = (uint32_t)qp * NS;  // scalars per orbital row
+  const int ntile = (int)(rs / 32);
+  const uint32_t vbytes = 8u * (uint32_t)nb * rs * ES;
+  const __amdgpu_buffer_rsrc_t bx = vec_rsrc(x, vbytes), by = vec_rsrc(y, vbytes);
+  const __amdgpu_buffer_rsrc_t bz = vec_rsrc(z != nullptr ? z : x, z != nullptr ? vbytes : 0u);
+  const S acx = (S)cx, acz = (S)cz;
+  const uint32_t half = 4u * rs * ES;  // orbital q -> q + 4
+  double dmm = 0.0, dm1 = 0.0;
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    const int i = __builtin_amdgcn_readfirstlane(base + g);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], deg = rp[i + 1] - p0;
+    S a[DM][4];
+    uint32_t o0[DM];
+#pragma unroll
+    for (int u = 0; u < DM; u++) {
+      if (deg > 0) {
+        const int pu = p0 + min(u, deg - 1);  // padding slots: the last block, weight 0
+        const int j = ci[pu];
+        mfma_a_of<S>(mfma_hraw<S>(hv + (size_t)pu * 128, m), m, u < deg ? S(1) : S(0), a[u]);
+        o0[u] = ((uint32_t)(8 * j + m.q) * rs + 2u * m.c) * ES;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[u][k] = S(0);
+        o0[u] = 0u;
+      }
+    }
+    const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 2u * m.c) * ES;
+    for (int t = 0; t < ntile; t++) {
+      const uint32_t so = (uint32_t)t * 32u * ES;
+      V4 acc = V4{S(0), S(0), S(0), S(0)};
+#pragma unroll
+      for (int u0 = 0; u0 < DM; u0 += G) {
+        C2 x0[G], x1[G];
+#pragma unroll
+        for (int v = 0; v < G; v++) {
+          x0[v] = IO::ld(bx, o0[u0 + v], so);
+          x1[v] = IO::ld(bx, o0[u0 + v] + half, so);
+        }
+#pragma unroll
+        for (int v = 0; v < G; v++) acc = mfma_block<S>(a[u0 + v], x0[v], x1[v], acc);
+      }
+      const C2 xo0 = IO::ld(bx, f0, so), xo1 = IO::ld(bx, f0 + half, so);
+      const C2 z0 = IO::ld(bz, f0, so), z1 = IO::ld(bz, f0 + half, so);  // 0 without z
+      C2 y0, y1;
+      y0.x = fma(acx, acc.x, acz * z0.x);
+      y0.y = fma(acx, acc.y, acz * z0.y);
+      y1.x = fma(acx, acc.z, acz * z1.x);
+      y1.y = fma(acx, acc.w, acz * z1.y);
+      if (rsrc.cr != 0.0) {
+        const int col = t * 16 + m.c;
+        mfma_rsource<S>(rsrc, (size_t)(8 * i + m.q), col, (f0 + so) / ES, y0);
+        mfma_rsource<S>(rsrc, (size_t)(8 * i + 4 + m.q), col, (f0 + half + so) / ES, y1);
+      }
+      IO::st(y0, by, f0, so);
+      IO::st(y1, by, f0 + half, so);
+      const double a0 = xo0.x, a1 = xo0.y, a2 = xo1.x, a3 = xo1.y;
+      dmm = fma(a0, a0, fma(a1, a1, fma(a2, a2, fma(a3, a3, dmm))));
+      dm1 = fma((double)y0.x, a0,
+                fma((double)y0.y, a1, fma((double)y1.x, a2, fma((double)y1.y, a3, dm1))));
+    }
+  }
+  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
+}
+
+// ---------------------------------------------------------------------------
+// Row-resident forward step with the next tile's loads in flight while the
+// current tile's MFMAs and epilogue run (two register buffers, tiles taken in
+// pairs).  Same contract and preconditions as mfma_step_rr_dev.
+// ---------------------------------------------------------------------------
+template <typename S, int DM = MF_DM>
+__device__ __forceinline__ void mfma_step_pp_dev(
+    WgId wid, const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
+  using C2 = typename Mfma<S>::c2;
+  using V4 = typename Mfma<S>::v4;
+  using IO = BufC2<S>;
+  constexpr int NS = Vec16<S>::N;
+  constexpr uint32_t ES = sizeof(S);
+  (void)nt;
+  const MfmaLane m = mfma_lane<S>();
+  const int g = threadIdx.x / WAVE;
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
+  const uint32_t rs = (uint32_t)qp * NS;
+  const int ntile = (int)(rs / 32);
+  const uint32_t vbytes = 8u * (uint32_t)nb * rs * ES;
+  const __amdgpu_buffer_rsrc_t bx = vec_rsrc(x, vbytes), by = vec_rsrc(y, vbytes);
+  const __amdgpu_buffer_rsrc_t bz = vec_rsrc(z != nullptr ? z : x, z != nullptr ? vbytes : 0u);
+  const S acx = (S)cx, acz = (S)cz;
+  const uint32_t half = 4u * rs * ES;
+  double dmm = 0.0, dm1 = 0.0;
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    const int i = __builtin_amdgcn_readfirstlane(base + g);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], deg = rp[i + 1] - p0;
+    S a[DM][4];
+    uint32_t o0[DM];
+#pragma unroll
+    for (int u = 0; u < DM; u++) {
+      if (deg > 0) {
+        const int pu = p0 + min(u, deg - 1);
+        const int j = ci[pu];
+        mfma_a_of<S>(mfma_hraw<S>(hv + (size_t)pu * 128, m), m, u < deg ? S(1) : S(0), a[u]);
+        o0[u] = ((uint32_t)(8 * j + m.q) * rs + 2u * m.c) * ES;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[u][k] = S(0);
+        o0[u] = 0u;
+      }
+    }
+    const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 2u * m.c) * ES;
+    auto load = [&](int t, C2 (&g0)[DM], C2 (&g1)[DM], C2 (&e)[4]) {
+      const uint32_t so = (uint32_t)t * 32u * ES;
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        g0[u] = IO::ld(bx, o0[u], so);
+        g1[u] = IO::ld(bx, o0[u] + half, so);
+      }
+      e[0] = IO::ld(bx, f0, so);
+      e[1] = IO::ld(bx, f0 + half, so);
+      e[2] = IO::ld(bz, f0, so);  // 0 without z
+      e[3] = IO::ld(bz, f0 + half, so);
+    };
+    auto finish = [&](int t, const C2 (&g0)[DM], const C2 (&g1)[DM], const C2 (&e)[4]) {
+      const uint32_t so = (uint32_t)t * 32u * ES;
+      V4 acc = V4{S(0), S(0), S(0), S(0)};
+#pragma unroll
+      for (int u = 0; u < DM; u++) acc = mfma_block<S>(a[u], g0[u], g1[u], acc);
+      C2 y0, y1;
+      y0.x = fma(acx, acc.x, acz * e[2].x);
+      y0.y = fma(acx, acc.y, acz * e[2].y);
+      y1.x = fma(acx, acc.z, acz * e[3].x);
+      y1.y = fma(acx, acc.w, acz * e[3].y);
+      if (rsrc.cr != 0.0) {
+        const int col = t * 16 + m.c;
+        mfma_rsource<S>(rsrc, (size_t)(8 * i + m.q), col, (f0 + so) / ES, y0);
+        mfma_rsource<S>(rsrc, (size_t)(8 * i + 4 + m.q), col, (f0 + half + so) / ES, y1);
+      }
+      IO::st(y0, by, f0, so);
+      IO::st(y1, by, f0 + half, so);
+      const double a0 = e[0].x, a1 = e[0].y, a2 = e[1].x, a3 = e[1].y;
+      dmm = fma(a0, a0, fma(a1, a1, fma(a2, a2, fma(a3, a3, dmm))));
+      dm1 = fma((double)y0.x, a0,
+                fma((double)y0.y, a1, fma((double)y1.x, a2, fma((double)y1.y, a3, dm1))));
+    };
+    C2 ga0[DM], ga1[DM], ea[4], gb0[DM], gb1[DM], eb[4];
+    load(0, ga0, ga1, ea);
+    for (int t = 0; t < ntile; t += 2) {
+      if (t + 1 < ntile) load(t + 1, gb0, gb1, eb);
+      finish(t, ga0, ga1, ea);
+      if (t + 1 >= ntile) break;
+      if (t + 2 < ntile) load(t + 2, ga0, ga1, ea);
+      finish(t + 1, gb0, gb1, eb);
+    }
+  }
+  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
+}
+
+// D(a, b) += fD * (G[a][b] + G[8+a][8+b], G[8+a][b] - G[a][8+b]) of block p from
+// the accumulator G in the MFMA output layout of S (see the file comment).
+template <typename S>
+__device__ __forceinline__ void mfma_flush_D(const dv4& G, const MfmaLane& m, double fD,
+                                             double* __restrict__ dblk) {
+  if constexpr (sizeof(S) == 4) {
+    // lane (q, c) holds G[4q + i][c]; lanes q < 2, c < 8 own D(a = 4q + i, b = c)
+    double dre[4], dim[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const double gv = G[e];
+      const double g40 = __shfl_xor(gv, 40);  // G[8 + a][8 + b]
+      const double g32 = __shfl_xor(gv, 32);  // G[8 + a][b]
+      const double g08 = __shfl_xor(gv, 8);   // G[a][8 + b]
+      dre[e] = gv + g40;
+      dim[e] = g32 - g08;
+    }
+    if (m.q < 2 && m.c < 8) {
+      double* d = dblk + (m.c * 8 + 4 * m.q) * 2;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        d[2 * e] = fma(fD, dre[e], d[2 * e]);
+        d[2 * e + 1] = fma(fD, dim[e], d[2 * e + 1]);
+      }
+    }
+  } else {
+    // lane (q, c) holds G[q + 4i][c]: G[a][c] (i = 0, 1), G[8 + a][c] (i = 2, 3)
+    double dre[2], dim[2];
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const double gr = G[e], gi = G[e + 2];
+      const double xr = __shfl_xor(gr, 8);  // G[a][8 + b]
+      const double xi = __shfl_xor(gi, 8);  // G[8 + a][8 + b]
+      dre[e] = gr + xi;
+      dim[e] = gi - xr;
+    }
+    if (m.c < 8) {
+      double* d = dblk + (m.c * 8 + m.q) * 2;
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        d[8 * e] = fma(fD, dre[e], d[8 * e]);
+        d[8 * e + 1] = fma(fD, dim[e], d[8 * e + 1]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row-resident autodiff step for rows of at most DM blocks (same contract as
+// mfma_autodiff_dev; the host uses it when every row has at most DM blocks
+// and a vector spans less than 4 GiB): A operands and neighbour offsets once
+// per row, buffer loads in the tile loop.
+// ---------------------------------------------------------------------------
+template <typename S, bool FUSED_D, int DM = MF_DM, int GR = 2>
+__device__ __forceinline__ void mfma_autodiff_rr_dev(
+    WgId wid, const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  using C2 = typename Mfma<S>::c2;
+  using V4 = typename Mfma<S>::v4;
+  using MF = Mfma<S>;
+  using IO = BufC2<S>;
+  constexpr int NS = Vec16<S>::N;
+  constexpr uint32_t ES = sizeof(S);
+  (void)nt;
+  const MfmaLane m = mfma_lane<S>();
+  const int g = threadIdx.x / WAVE;
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
+  const uint32_t rs = (uint32_t)qp * NS;
+  const int ntile = (int)(rs / 32);
+  const uint32_t vbytes = 8u * (uint32_t)nb * rs * ES;
+  const bool vb1 = (bm1 == nullptr);
+  const double hb1 = vb1 ? sb : 1.0;
+  const S* xbp = vb1 ? am : bm1;  // gathered beta_{m+1} (or alpha_m, scaled by sb)
+  const __amdgpu_buffer_rsrc_t ba = vec_rsrc(am, vbytes), bb = vec_rsrc(xbp, vbytes);
+  const __amdgpu_buffer_rsrc_t ba1 = vec_rsrc(am1, vbytes);
+  const __amdgpu_buffer_rsrc_t bb2 = vec_rsrc(bm2 != nullptr ? bm2 : am1, vbytes);
+  const __amdgpu_buffer_rsrc_t bao = vec_rsrc(a_out, vbytes), bbo = vec_rsrc(b_out, vbytes);
+  const S a2 = (S)2, ab2 = (S)(2.0 * hb1), ac4 = (S)c4, ac2 = (S)c2, aw = (S)w;
+  const S asb2 = bm2 != nullptr ? S(1) : (S)sb2;  // beta_{m+2} = asb2 * (bm2 or alpha_{m+1})
+  const double fD = dscale * hb1;
+  const int dorb = m.c & 7, dpart = m.c >> 3;
+  const uint32_t half = 4u * rs * ES;
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    const int i = __builtin_amdgcn_readfirstlane(base + g);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], deg = rp[i + 1] - p0;
+    S a[DM][4];
+    uint32_t o0[DM], od[DM];
+#pragma unroll
+    for (int u = 0; u < DM; u++) {
+      if (deg > 0) {
+        const int pu = p0 + min(u, deg - 1);
+        const int j = ci[pu];
+        mfma_a_of<S>(mfma_hraw<S>(hv + (size_t)pu * 128, m), m, u < deg ? S(1) : S(0), a[u]);
+        o0[u] = ((uint32_t)(8 * j + m.q) * rs + 2u * m.c) * ES;
+        od[u] = ((uint32_t)(8 * j + dorb) * rs + (uint32_t)dpart + 2u * m.q) * ES;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[u][k] = S(0);
+        o0[u] = od[u] = 0u;
+      }
+    }
+    const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 2u * m.c) * ES;
+    const uint32_t fd = ((uint32_t)(8 * i + dorb) * rs + (uint32_t)dpart + 2u * m.q) * ES;
+    dv4 G[DM];
+#pragma unroll
+    for (int u = 0; u < DM; u++) G[u] = dv4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < ntile; t++) {
+      const uint32_t so = (uint32_t)t * 32u * ES;
+      V4 accA = V4{S(0), S(0), S(0), S(0)}, accB = accA;
+      S pv[4];
+      if constexpr (FUSED_D) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) pv[k] = IO::ld1(ba, fd + 8u * k * ES, so);
+      }
+#pragma unroll
+      for (int u0 = 0; u0 < DM; u0 += GR) {
+        C2 xa0[GR], xa1[GR], xb0[GR], xb1[GR];
+        S qv[GR][4];
+#pragma unroll
+        for (int v = 0; v < GR; v++) {
+          xa0[v] = IO::ld(ba, o0[u0 + v], so);
+          xa1[v] = IO::ld(ba, o0[u0 + v] + half, so);
+          xb0[v] = IO::ld(bb, o0[u0 + v], so);
+          xb1[v] = IO::ld(bb, o0[u0 + v] + half, so);
+          if constexpr (FUSED_D) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) qv[v][k] = IO::ld1(bb, od[u0 + v] + 8u * k * ES, so);
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < GR; v++) {
+          accA = mfma_block<S>(a[u0 + v], xa0[v], xa1[v], accA);
+          accB = mfma_block<S>(a[u0 + v], xb0[v], xb1[v], accB);
+          if constexpr (FUSED_D) {
+            V4 gt = V4{S(0), S(0), S(0), S(0)};
+#pragma unroll
+            for (int k = 0; k < 4; k++) gt = MF::mac(pv[k], qv[v][k], gt);
+            G[u0 + v] += dv4{(double)gt.x, (double)gt.y, (double)gt.z, (double)gt.w};
+          }
+        }
+      }
+      const C2 ai0 = IO::ld(ba, f0, so), ai1 = IO::ld(ba, f0 + half, so);
+      const C2 a10 = IO::ld(ba1, f0, so), a11 = IO::ld(ba1, f0 + half, so);
+      const C2 b20 = IO::ld(bb2, f0, so), b21 = IO::ld(bb2, f0 + half, so);
+      const S hx[4] = {accA.x, accA.y, accA.z, accA.w};
+      const S hb[4] = {accB.x, accB.y, accB.z, accB.w};
+      const S ai[4] = {ai0.x, ai0.y, ai1.x, ai1.y};
+      const S a1[4] = {a10.x, a10.y, a11.x, a11.y};
+      const S b2[4] = {asb2 * b20.x, asb2 * b20.y, asb2 * b21.x, asb2 * b21.y};
+      S an[4], bo[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        an[e] = fma(a2, hx[e], -a1[e]);
+        S v = fma(ab2, hb[e], -b2[e]);
+        v = fma(ac4, ai[e], v);
+        v = fma(ac2, a1[e], v);
+        v = fma(aw, an[e], v);
+        bo[e] = v;
+      }
+      IO::st(C2{an[0], an[1]}, bao, f0, so);
+      IO::st(C2{an[2], an[3]}, bao, f0 + half, so);
+      IO::st(C2{bo[0], bo[1]}, bbo, f0, so);
+      IO::st(C2{bo[2], bo[3]}, bbo, f0 + half, so);
+    }
+    if constexpr (FUSED_D) {
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        if (u >= deg) break;
+        mfma_flush_D<S>(G[u], m, fD, dacc + (size_t)(p0 + u) * 128);
+      }
+    }
+  }
 }
 
 // Neighbour blocks of one pass of the autodiff kernel: the FP64 G
@@ -338,7 +759,7 @@ __device__ __forceinline__ void mfma_autodiff_dev(
             const int p = pc + u0 + v;
             if (p < pe) {
               const int j = ci[p];
-              mfma_a<S>(hv + (size_t)p * 128, m, a[v]);
+              mfma_a_of<S>(mfma_hraw<S>(hv + (size_t)p * 128, m), m, S(1), a[v]);
               const size_t r0 = (size_t)(8 * j + m.q) * rs + co, r1 = (size_t)(8 * j + 4 + m.q) * rs + co;
               xa0[v] = ldc<S>(am + r0);
               xa1[v] = ldc<S>(am + r1);
@@ -473,6 +894,26 @@ __global__ __launch_bounds__(WG) void k_mfma_autodiff(
     double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
     double sb, double sb2, double dscale, int nt) {
   mfma_autodiff_dev<S, FUSED_D>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out, dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_mfma_step_rr(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
+  mfma_step_rr_dev<S, MF_DM, 4>(wg_self(), rp, ci, hv, x, z, y, nb, qp, chunk, cx, cz, partial, nt,
+                                rsrc);
+}
+
+template <typename S>
+__global__ __launch_bounds__(WG) void k_mfma_autodiff_rr(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  mfma_autodiff_rr_dev<S, true, MF_DM, 1>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out,
+                                          dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
 }
 
 }  // namespace fkd

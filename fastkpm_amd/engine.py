@@ -276,7 +276,7 @@ class HipEngine:
         mf = C.c_int()
         call("fkpm_get_kernel_mfma", self._h, int(kind), C.byref(mf))
         return {"lanes": v[0].value, "kq": v[1].value, "n_wg": v[2].value, "nt": v[3].value,
-                "mfma": bool(mf.value)}
+                "mfma": bool(mf.value), "mfma_variant": mf.value}
 
     def state(self) -> dict:
         v = [C.c_int() for _ in range(3)]

@@ -164,8 +164,9 @@ int fkpm_autotune(fkpm_handle* h);
 /* Geometry in use for one kernel kind. */
 int fkpm_get_kernel_config(fkpm_handle* h, int kind, int* lanes, int* kq, int* n_wg, int* nt);
 
-/* *mfma = 1 if that kernel kind runs on the matrix cores.  No counterpart in
- * the reference. */
+/* *mfma = 0 if that kernel kind runs on the vector units, 1 on the matrix
+ * cores, 2 on the matrix cores in the row-resident form (rows of at most 8
+ * blocks).  No counterpart in the reference. */
 int fkpm_get_kernel_mfma(fkpm_handle* h, int kind, int* mfma);
 
 /* r_compact: R has at most one nonzero per row (probing colours) and the

@@ -77,12 +77,18 @@ def test_mfma_fp64_matches_oracle(oracle, case):
     eng, orc, lo, hi = _setup(oracle, H, R)
     eng.moments(2)
     assert eng.kernel_config(K_MOM)["mfma"] and eng.kernel_config(K_AD)["mfma"]
+    # rows of at most 8 blocks take the row-resident kernels, longer rows the general ones
+    variant = 2 if case == "lattice" else 1
+    assert eng.kernel_config(K_MOM)["mfma_variant"] == variant
+    assert eng.kernel_config(K_AD)["mfma_variant"] == variant
     _check_all(oracle, eng, orc, H, lo, hi, 64, FP64_TOL, FP64_TOL)
 
 
-def test_mfma_equals_vector_kernels(oracle):
-    """The matrix-core and vector-unit paths agree with each other and the oracle."""
-    H = _dense_rows_b8(nb=16, extra=5, seed=9)
+@pytest.mark.parametrize("kind", ["dense_rows", "lattice"])
+def test_mfma_equals_vector_kernels(oracle, kind):
+    """The matrix-core (general and row-resident) and vector-unit paths agree
+    with each other."""
+    H = _dense_rows_b8(nb=16, extra=5, seed=9) if kind == "dense_rows" else _lattice_b8(seed=8)
     R = _probing_R(oracle, H, seed=5)
     out = {}
     for mf in (True, False):
@@ -153,7 +159,7 @@ def test_mfma_fp32_mixed(oracle):
     c = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
     mu = eng.moments(M)
     mu_o = orc.moments(M)
-    assert eng.kernel_config(K_MOM)["mfma"]
+    assert eng.kernel_config(K_MOM)["mfma_variant"] == 2
     assert rel_err(mu, mu_o) < 2e-5
     e, e_o = float(np.dot(c, mu)), float(np.dot(c, mu_o))
     assert abs(e - e_o) / abs(e_o) < 1e-6
