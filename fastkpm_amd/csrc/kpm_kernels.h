@@ -302,12 +302,16 @@ __device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step, WgId wi
 
 // Block row i's gather: acc[x][k][bi] += sum_p Hs_p(bi, bj) X_x[B j_p + bj]
 // over the blocks p0..p1 of the row, for NX vectors sharing the pattern, then
-// blk(p, xv) with the gathered packets xv[x][bj][k][.] of block p.  qo is the
+// blk(p, j_p, xv) with the gathered packets xv[x][bj][k][.] of block p.  qo is the
 // scalar offset of this lane's first packet in a row.
-template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, class F>
-__device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* __restrict__ hv,
-                                           const S* const (&X)[NX], size_t rs, size_t qo, int p0,
-                                           int p1, A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk) {
+// With OWN, the packets of X[0] in the row's own block (column iown) are taken
+// from `own` (already in registers) instead of being loaded again.
+template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, bool OWN, class F>
+__device__ __forceinline__ void gather_row_impl(const int* __restrict__ ci,
+                                                const S* __restrict__ hv, const S* const (&X)[NX],
+                                                size_t rs, size_t qo, int p0, int p1,
+                                                A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk,
+                                                int iown, const A (&own)[KQ][B][Vec16<S>::N]) {
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS, HS = GE::HS, BBS = GE::BBS;
   constexpr int U = GE::template batch<NX>();
@@ -321,13 +325,23 @@ __device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* 
     for (int u = 0; u < U; u++) {
       if (j[u] >= 0) {
 #pragma unroll
-        for (int x = 0; x < NX; x++)
+        for (int x = 0; x < NX; x++) {
+          if (OWN && x == 0 && j[u] == iown) {
+#pragma unroll
+            for (int bj = 0; bj < B; bj++)
+#pragma unroll
+              for (int k = 0; k < KQ; k++)
+#pragma unroll
+                for (int e = 0; e < NS; e++) xv[u][0][bj][k][e] = own[k][bj][e];
+            continue;
+          }
 #pragma unroll
           for (int bj = 0; bj < B; bj++)
 #pragma unroll
             for (int k = 0; k < KQ; k++)
               ldv<S, NS>(X[x] + (size_t)(B * j[u] + bj) * rs + qo + (size_t)k * L * NS,
                          xv[u][x][bj][k]);
+        }
 #pragma unroll
         for (int t = 0; t < BBS; t++) hh[u][t] = (A)hv[(size_t)(p + u) * BBS + t];
       } else {
@@ -358,9 +372,17 @@ __device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* 
             for (int bi = 0; bi < B; bi++)
               cmac<CX, NS>(acc[x][k][bi], hh[u][(bj * B + bi) * HS],
                            CX ? hh[u][(bj * B + bi) * HS + HS - 1] : A(0), xv[u][x][bj][k]);
-      blk(p + u, xv[u]);
+      blk(p + u, j[u], xv[u]);
     }
   }
+}
+
+template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, class F>
+__device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* __restrict__ hv,
+                                           const S* const (&X)[NX], size_t rs, size_t qo, int p0,
+                                           int p1, A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk) {
+  const A none[KQ][B][Vec16<S>::N] = {};
+  gather_row_impl<S, A, CX, B, L, KQ, NX, false>(ci, hv, X, rs, qo, p0, p1, acc, blk, -1, none);
 }
 
 // Own rows of block row i: v[k][bi] = X[B i + bi] packets of this lane.
@@ -445,14 +467,28 @@ __device__ __forceinline__ void step_moments_dev(
       const size_t qo = (size_t)(q0 + l) * NS;
       A xd[KQ][B][NS];
       A acc[1][KQ][B][NS];
-      load_rows<S, A, B, L, KQ>(x, i, rs, qo, xd);
+      // the own rows of x (for the dots) arrive with the diagonal block of
+      // the gather; every row has one (set_H requires it)
 #pragma unroll
       for (int k = 0; k < KQ; k++)
 #pragma unroll
         for (int bi = 0; bi < B; bi++)
 #pragma unroll
-          for (int e = 0; e < NS; e++) acc[0][k][bi][e] = A(0);
-      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [](int, auto&) {});
+          for (int e = 0; e < NS; e++) {
+            acc[0][k][bi][e] = A(0);
+            xd[k][bi][e] = A(0);
+          }
+      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc,
+                                         [&](int, int jb, auto& xv) {
+                                           if (jb != i) return;
+#pragma unroll
+                                           for (int k = 0; k < KQ; k++)
+#pragma unroll
+                                             for (int bi = 0; bi < B; bi++)
+#pragma unroll
+                                               for (int e = 0; e < NS; e++)
+                                                 xd[k][bi][e] = xv[0][bi][k][e];
+                                         });
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
 #pragma unroll
@@ -527,7 +563,7 @@ __global__ __launch_bounds__(WG) void k_step_xi(
         for (int bi = 0; bi < B; bi++)
 #pragma unroll
           for (int e = 0; e < NS; e++) acc[0][k][bi][e] = A(0);
-      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [](int, auto&) {});
+      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [](int, int, auto&) {});
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
 #pragma unroll
@@ -613,8 +649,8 @@ __device__ __forceinline__ void step_autodiff_dev(
           for (int bi = 0; bi < B; bi++)
 #pragma unroll
             for (int e = 0; e < NS; e++) acc[x][k][bi][e] = A(0);
-      gather_row<S, A, CX, B, L, KQ, 2>(
-          ci, hv, X, rs, qo, p0, p1, acc, [&](int p, auto& xv) {
+      gather_row_impl<S, A, CX, B, L, KQ, 2, true>(
+          ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int, auto& xv) {
             constexpr int NV = GE::DE * HS;  // scalars of one D block
             if constexpr (FUSED_D && NV <= 16) {
               // D_p(bi, bj) += fD * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s]):
@@ -688,7 +724,8 @@ __device__ __forceinline__ void step_autodiff_dev(
                 }
               }
             }
-          });
+          },
+          i, ai);
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
 #pragma unroll
