@@ -17,7 +17,9 @@ on the same workload with M' moments (a bounded sample) and reports N*M'*S/s.
 The "serving" entry runs a seeded Poisson stream of mixed requests (energy +
 dE/dH each) through the batched engine, FP64 and FP32-mixed, and reports
 p50/p99 latency and aggregate updates/s; --serve-only runs that stream alone,
-at the full request ranges, and prints it as its own JSON line.
+at the full request ranges, and prints it as its own JSON line.  The
+"conductivity" entry times the two-current moments moments2_v1(M=128) of the
+x current on the same workload (FP64 and FP32-mixed) and the static sigma_xx.
 """
 from __future__ import annotations
 
@@ -316,6 +318,40 @@ def serving_leg(n_req: int, n_max: int, m_max: int, rate: float, seed: int, hbm_
     return out
 
 
+def conductivity_leg(H, groups, cfg, es, seed: int, M2: int, reps: int = 2) -> dict:
+    """Two-current moments (moments2_v1) of the x current on the same workload
+    and the static conductivity sigma_xx from them, FP64 and FP32-mixed."""
+    shape = cfg["shape"]
+    jx = lattice.current(H, shape, 0)
+    N = H.n_rows * H.b_len
+    c2 = host.conductivity_coefficients(M2, 2 * M2, max(cfg.get("kT", 0.01), 1e-3),
+                                        cfg.get("mu", 0.0), 0.0, es.lo, es.hi)
+    out = {"M": M2, "current": "j_x = i[H, X] (bonds along x)", "reps": reps}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        eng = HipEngine("cx_double", prec)
+        eng.set_R_correlated(groups, host.RNG(seed))
+        eng.set_H(H, es)
+        S = eng.R.shape[1]
+        eng.moments2_v1(M2, jx, jx)  # warm-up: buffers, rocBLAS handle
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mu2 = eng.moments2_v1(M2, jx, jx)
+        t = (time.perf_counter() - t0) / reps
+        eng.close()
+        # complex GEMM: 8 real flops per multiply-add, M^2 N S of them; the
+        # recursions: 2 M products with H and M with j2, one with j1
+        gemm_flops = 8.0 * M2 * M2 * N * S
+        res[prec] = mu2
+        out[prec] = {"ms": round(1e3 * t, 2), "sigma_xx": complex(np.sum(c2 * mu2)).real,
+                     "gemm_TFLOPs_if_all_time": round(gemm_flops / t / 1e12, 2),
+                     "updates_per_s": 2 * M2 * N * S / t}
+    out["fp32_mu2_rel_err_vs_fp64"] = float(np.max(np.abs(res["fp32"] - res["fp64"]))
+                                            / np.max(np.abs(res["fp64"])))
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +364,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-mixed comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-conductivity", action="store_true",
+                    help="skip the two-current (conductivity) moments")
+    ap.add_argument("--conductivity-moments", type=int, default=128)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving stream, at the full request ranges")
     ap.add_argument("--serve-requests", type=int, default=16)
@@ -449,6 +488,10 @@ def main(argv=None) -> int:
         _log("serving stream")
         out["serving"] = serving_leg(args.serve_requests, 1 << 18, 1024, args.serve_rate,
                                      args.seed, args.serve_hbm_gb)
+    if not args.no_conductivity and args.precision == "fp64" and "shape" in cfg:
+        _log(f"two-current moments, M={args.conductivity_moments}")
+        out["conductivity"] = conductivity_leg(H, groups, cfg, r["es"], args.seed,
+                                               args.conductivity_moments)
     _log("exact-diagonalisation check")
     out["exact_check"] = exact_check(args.config, args.precision, args.seed)
     if not args.no_cpu_baseline:

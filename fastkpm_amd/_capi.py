@@ -38,6 +38,7 @@ SIGNATURES = {
     "fkpm_stoch_matrix": (_I, [_P, _I, _DP, _P]),
     "fkpm_autodiff_matrix": (_I, [_P, _I, _DP, _P]),
     "fkpm_get_D": (_I, [_P, _P]),
+    "fkpm_moments2": (_I, [_P, _I, _I, _IP, _IP, _P, _I, _IP, _IP, _P, _I, _DP]),
     "fkpm_set_timing": (_I, [_P, _I]),
     "fkpm_get_timing": (_I, [_P, _DP, _IP, _DP, _IP, _DP, _IP]),
     "fkpm_get_config": (_I, [_P, _IP, _IP, _IP, _IP, _IP]),

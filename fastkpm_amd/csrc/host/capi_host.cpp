@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "fastkpm.h"
@@ -51,12 +52,18 @@ void copy_R(RGen<T> const& g, void* out) {
   std::memcpy(out, g.R.memptr(), g.R.size() * sizeof(T));
 }
 
+thread_local std::string g_host_error;
+
 template <typename F>
 int guarded(F&& f) {
   try {
     f();
     return 0;
+  } catch (std::exception const& e) {
+    g_host_error = e.what();
+    return FKPM_E_ARG;
   } catch (...) {
+    g_host_error = "unknown exception";
     return FKPM_E_ARG;
   }
 }
@@ -122,6 +129,8 @@ struct CorF {
 
 extern "C" {
 
+const char* fkpm_host_last_error(void) { return g_host_error.c_str(); }
+
 void* fkpm_host_rng_new(uint32_t seed) { return new fkpm::RNG(seed); }
 void fkpm_host_rng_free(void* rng) { delete static_cast<fkpm::RNG*>(rng); }
 uint32_t fkpm_host_rng_next(void* rng) { return (uint32_t)(*static_cast<fkpm::RNG*>(rng))(); }
@@ -181,6 +190,39 @@ int fkpm_host_set_R_probing(int n, const int* groups, void* rng, int* rcol_out, 
       }
     }
     if (s_out) *s_out = s;
+  });
+}
+
+int fkpm_host_conductivity_coefficients(int M, int Mq, double kT, double mu, double omega,
+                                        double lo, double hi, const double* kernel, double* out) {
+  return guarded([&] {
+    if (M <= 0 || !kernel || !out) throw std::invalid_argument("conductivity: bad arguments");
+    fkpm::Vec<double> k(kernel, kernel + M);
+    const auto c = fkpm::electrical_conductivity_coefficients(M, Mq, kT, mu, omega,
+                                                              fkpm::EnergyScale{lo, hi}, k);
+    for (int a = 0; a < M; a++)
+      for (int b = 0; b < M; b++) {
+        out[((size_t)a * M + b) * 2] = c[a][b].real();
+        out[((size_t)a * M + b) * 2 + 1] = c[a][b].imag();
+      }
+  });
+}
+
+int fkpm_host_moment_transform2(int M, const double* mu, int Mq, const double* kernel,
+                                double* gamma_out) {
+  return guarded([&] {
+    if (M <= 0 || Mq <= 0 || !mu || !kernel || !gamma_out)
+      throw std::invalid_argument("moment_transform2: bad arguments");
+    fkpm::Vec<fkpm::Vec<fkpm::cx_double>> m(M, fkpm::Vec<fkpm::cx_double>(M));
+    for (int a = 0; a < M; a++)
+      for (int b = 0; b < M; b++)
+        m[a][b] = fkpm::cx_double(mu[((size_t)a * M + b) * 2], mu[((size_t)a * M + b) * 2 + 1]);
+    const auto g = fkpm::moment_transform(m, Mq, fkpm::Vec<double>(kernel, kernel + M));
+    for (int i = 0; i < Mq; i++)
+      for (int j = 0; j < Mq; j++) {
+        gamma_out[((size_t)i * Mq + j) * 2] = g[i][j].real();
+        gamma_out[((size_t)i * Mq + j) * 2 + 1] = g[i][j].imag();
+      }
   });
 }
 

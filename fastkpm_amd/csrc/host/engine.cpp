@@ -186,9 +186,23 @@ class Engine_HIP : public Engine<T> {
     return mu;
   }
 
-  Vec<Vec<cx_double>> moments2_v1(int, SpMatBsr<T> const&, SpMatBsr<T> const&, int, int) override {
-    std::cerr << "moments2_v1 is not provided by the HIP engine\n";
-    std::exit(EXIT_FAILURE);
+  // Two-current moments on the device (src/engine_cusparse.cpp:793-902).
+  Vec<Vec<cx_double>> moments2_v1(int M, SpMatBsr<T> const& j1op, SpMatBsr<T> const& j2op,
+                                  int a_chunk_ncols, int R_chunk_ncols) override {
+    (void)a_chunk_ncols;  // every Chebyshev order stays in HBM
+    for (const SpMatBsr<T>* j : {&j1op, &j2op})
+      if (j->n_rows != j->n_cols || j->b_len <= 0)
+        throw std::invalid_argument("moments2_v1: current operators must be square BSR matrices");
+    Vec<double> flat((size_t)M * M * 2);
+    check_rc(fkpm_moments2(h_, M, j1op.n_blocks(), j1op.row_ptr.data(), j1op.col_idx.data(),
+                           j1op.val.data(), j2op.n_blocks(), j2op.row_ptr.data(),
+                           j2op.col_idx.data(), j2op.val.data(), R_chunk_ncols, flat.data()),
+             "moments2_v1");
+    Vec<Vec<cx_double>> mu(M, Vec<cx_double>(M));
+    for (int a = 0; a < M; a++)
+      for (int b = 0; b < M; b++)
+        mu[a][b] = cx_double(flat[((size_t)a * M + b) * 2], flat[((size_t)a * M + b) * 2 + 1]);
+    return mu;
   }
   Vec<Vec<cx_double>> moments2_v2(int, SpMatBsr<T> const&, SpMatBsr<T> const&, int, int) override {
     std::cerr << "moments2_v2 is not provided by the HIP engine\n";

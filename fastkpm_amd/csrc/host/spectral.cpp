@@ -75,6 +75,120 @@ Vec<double> moment_transform(Vec<double> const& moments, int Mq) {
   return gamma;
 }
 
+// src/fastkpm.cpp:80-233, the direct-sum branch (no FFTW in this build):
+// static (omega = 0) Kubo-Bastin coefficients from the Chebyshev nodes with
+// 1 - x^2 >= 1e-4, or the optical ones from the nodes i >= i_start; both
+// times (2 - delta_m1,0)(2 - delta_m2,0) g_m1 g_m2 * 2 pi / (Mq mag^2).
+Vec<Vec<cx_double>> electrical_conductivity_coefficients(int M, int Mq, double kT, double mu,
+                                                         double omega, EnergyScale es,
+                                                         Vec<double> const& kernel) {
+  if ((int)kernel.size() != M) throw std::invalid_argument("conductivity: kernel size != M");
+  if (Mq < 2 * M) throw std::invalid_argument("conductivity: Mq must be >= 2 M");
+  if (omega < 0.0) throw std::invalid_argument("conductivity: omega must be >= 0");
+  constexpr double cutoff = 1e-4;
+  const double ws = omega / es.mag();
+  Vec<Vec<cx_double>> ret(M, Vec<cx_double>(M, cx_double(0.0, 0.0)));
+  if (ws >= 2.0) return ret;
+  Vec<double> Ti(M), Tj(M);
+  if (ws < 1.0e-10) {
+    for (int i = 0; i < Mq; i++) {
+      const double x_i = std::cos(Pi * (i + 0.5) / Mq);
+      if (1.0 - x_i * x_i < cutoff) continue;
+      chebyshev_fill_array(x_i, Ti);     // T_m(x) = cos(m t)
+      chebyshev_fill_array(x_i, Tj, 2);  // U_m(x)
+      const double s1 = std::sqrt(1.0 - x_i * x_i);
+      for (int m = M - 1; m > 0; m--) Tj[m] = Tj[m - 1] * s1;  // sin(m t)
+      Tj[0] = 0.0;
+      const double f_i = fermi_density(es.unscale(x_i), kT, mu) / std::pow(1.0 - x_i * x_i, 1.5);
+      for (int m1 = 0; m1 < M; m1++)
+        for (int m2 = 0; m2 < M; m2++)
+          ret[m1][m2] += (Ti[m1] * cx_double(Ti[m2], -Tj[m2]) * cx_double(x_i, m2 * s1) +
+                          Ti[m2] * cx_double(Ti[m1], Tj[m1]) * cx_double(x_i, -m1 * s1)) *
+                         f_i;
+    }
+  } else {
+    const int i_start = (int)std::ceil(std::acos(1.0 - ws) / Pi * Mq - 0.5);
+    if (Mq - i_start < 20)
+      throw std::invalid_argument("conductivity: fewer than 20 nodes below 1 - omega");
+    for (int i = i_start; i < Mq; i++) {
+      const double x_i = std::cos(Pi * (i + 0.5) / Mq);
+      const double xw = x_i + ws;
+      if (x_i < -1.0 + cutoff || xw > 1.0 - cutoff) continue;
+      chebyshev_fill_array(xw, Ti);   // T_m(x + omega)
+      chebyshev_fill_array(x_i, Tj);  // T_m(x)
+      const double s2 = std::sqrt(1.0 - xw * xw);
+      const double f_i =
+          (fermi_density(es.unscale(x_i), kT, mu) - fermi_density(es.unscale(x_i) + omega, kT, mu)) /
+          (ws * s2);
+      for (int m1 = 0; m1 < M; m1++)
+        for (int m2 = 0; m2 < M; m2++) ret[m1][m2] += Ti[m1] * Tj[m2] * f_i;
+    }
+  }
+  for (int m1 = 0; m1 < M; m1++) {
+    // the factor 2 pi changes the unit from e^2/hbar to e^2/h
+    const double t1 = 2.0 * Pi * (m1 == 0 ? 1.0 : 2.0) * kernel[m1] / (Mq * es.mag() * es.mag());
+    for (int m2 = 0; m2 < M; m2++) ret[m1][m2] *= t1 * (m2 == 0 ? 1.0 : 2.0) * kernel[m2];
+  }
+  return ret;
+}
+
+// src/fastkpm.cpp:300-312
+cx_double moment_product(Vec<Vec<cx_double>> const& c, Vec<Vec<cx_double>> const& mu) {
+  if (c.size() != mu.size() || (c.size() > 0 && c[0].size() != mu[0].size()))
+    throw std::invalid_argument("moment_product: shapes differ");
+  cx_double r(0.0, 0.0);
+  for (size_t a = 0; a < c.size(); a++)
+    for (size_t b = 0; b < c[a].size(); b++) r += c[a][b] * mu[a][b];
+  return r;
+}
+
+// src/fastkpm.cpp:258-289: gamma[i][j] = sum T_m1(x_i) T_m2(x_j) mup[m1][m2],
+// evaluated as two matrix products over the node table T[i][m] = T_m(x_i).
+Vec<Vec<cx_double>> moment_transform(Vec<Vec<cx_double>> const& moments, int Mq,
+                                     Vec<double> const& kernel) {
+  const int M = (int)moments.size();
+  if ((int)kernel.size() != M) throw std::invalid_argument("moment_transform: kernel size != M");
+  Vec<Vec<cx_double>> mup(M, Vec<cx_double>(M));
+  for (int a = 0; a < M; a++)
+    for (int b = 0; b < M; b++)
+      mup[a][b] = (a == 0 ? 1.0 : 2.0) * (b == 0 ? 1.0 : 2.0) * kernel[a] * kernel[b] * moments[a][b];
+  Vec<Vec<double>> T(Mq, Vec<double>(M));
+  for (int i = 0; i < Mq; i++) chebyshev_fill_array(std::cos(Pi * (i + 0.5) / Mq), T[i]);
+  Vec<Vec<cx_double>> G(Mq, Vec<cx_double>(M, cx_double(0.0, 0.0)));  // G = T mup
+  for (int i = 0; i < Mq; i++)
+    for (int a = 0; a < M; a++)
+      for (int b = 0; b < M; b++) G[i][b] += T[i][a] * mup[a][b];
+  Vec<Vec<cx_double>> gamma(Mq, Vec<cx_double>(Mq, cx_double(0.0, 0.0)));  // gamma = G T^t
+  for (int i = 0; i < Mq; i++)
+    for (int j = 0; j < Mq; j++) {
+      cx_double g(0.0, 0.0);
+      for (int b = 0; b < M; b++) g += G[i][b] * T[j][b];
+      gamma[i][j] = g;
+    }
+  return gamma;
+}
+
+// src/fastkpm.cpp:335-360
+void density_function(Vec<Vec<cx_double>> const& gamma, EnergyScale es, Vec<double>& x,
+                      Vec<double>& y, Vec<Vec<cx_double>>& rho) {
+  const int Mq1 = (int)gamma.size();
+  const int Mq2 = Mq1 > 0 ? (int)gamma[0].size() : 0;
+  x.resize(Mq1);
+  y.resize(Mq2);
+  rho.assign(Mq1, Vec<cx_double>(Mq2, cx_double(0.0, 0.0)));
+  for (int i2 = Mq2 - 1; i2 >= 0; i2--) y[Mq2 - 1 - i2] = es.unscale(std::cos(Pi * (i2 + 0.5) / Mq2));
+  for (int i1 = Mq1 - 1; i1 >= 0; i1--) {
+    const double x1 = std::cos(Pi * (i1 + 0.5) / Mq1);
+    x[Mq1 - 1 - i1] = es.unscale(x1);
+    for (int i2 = Mq2 - 1; i2 >= 0; i2--) {
+      const double x2 = std::cos(Pi * (i2 + 0.5) / Mq2);
+      rho[Mq1 - 1 - i1][Mq2 - 1 - i2] =
+          gamma[i1][i2] /
+          (Pi * std::sqrt(1.0 - x1 * x1) * Pi * std::sqrt(1.0 - x2 * x2) * es.mag() * es.mag());
+    }
+  }
+}
+
 // src/fastkpm.cpp:314-322
 double density_product(Vec<double> const& gamma, std::function<double(double)> f, EnergyScale es) {
   const int Mq = (int)gamma.size();

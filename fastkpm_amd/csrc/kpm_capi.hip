@@ -19,12 +19,14 @@
 #include <string>
 #include <vector>
 
+
 #include "fastkpm_hip.h"
 #include "kpm_dispatch.h"
 #include "kpm_internal.h"
 // this unit defines the non-template helper kernels
 #define FKD_DEFINE_HELPERS
 #include "kpm_kernels.h"
+#include "kpm_dots.h"
 
 namespace {
 
@@ -203,6 +205,9 @@ struct fkpm_handle {
   DevBuf drp, dci, tpos, dacc, dout;
   std::vector<int> hd_rp, hd_ci;
 
+  // --- two-current moments (moments2) ---
+  DevBuf j_rp[2], j_ci[2], j_rb[2], j_raw[2], j_val[2], m2_a, m2_b, m2_ring, m2_c, m2_part;
+
   // --- timing ---
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -217,6 +222,9 @@ struct fkpm_handle {
                       &xi,    &partial, &psum,  &tune_dacc, &drp, &dci,  &tpos,     &dacc,
                       &dout};
     for (DevBuf* d : bufs) d->release();
+    DevBuf* m2bufs[] = {&j_rp[0], &j_ci[0], &j_rb[0], &j_raw[0], &j_val[0], &j_rp[1], &j_ci[1],
+                        &j_rb[1], &j_raw[1], &j_val[1], &m2_a, &m2_b, &m2_ring, &m2_c, &m2_part};
+    for (DevBuf* d : m2bufs) d->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -870,6 +878,238 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Two-current moments (Engine_CPU::moments2_v1, engine_cpu.cpp:86-160;
+// engine_cusparse.cpp:793-902):
+//   mu[m1][m2] = sum_k <alpha_m1(k), j2 atild_m2(k)>,
+//   alpha_m(k) = T_m(Hs) r_k,  atild_m(k) = T_m(Hs) j1 r_k.
+// For a chunk of probe columns all M alpha_m and all M j2 atild_m are kept in
+// HBM as the columns of two K x M matrices (K = n * padded columns), built by
+// the step kernels (j1, j2 on their own block patterns), and the M x M block
+// of dots A^H B is formed on the matrix cores (kpm_dots.h), accumulated in
+// FP64 over chunks.
+// ---------------------------------------------------------------------------
+
+// The kernel set and geometry for a pass width of W packets (a compiled
+// (L, KQ) with L * KQ == W, widest lane groups first).
+template <class S>
+bool kset_for_width(fkpm_handle* h, int W, fkd::KSet<S>* ks, int* L, int* KQ);
+template <>
+bool kset_for_width<double>(fkpm_handle* h, int W, fkd::KSet<double>* ks, int* L, int* KQ) {
+  fkd::GeomChoice gs[64];
+  const int ng = fkd::geometries_for_block(h->b, gs, 64);
+  for (int want = 64; want >= 1; want /= 2)
+    for (int k = 0; k < ng; k++)
+      if (gs[k].L == want && gs[k].L * gs[k].KQ == W) {
+        *L = gs[k].L;
+        *KQ = gs[k].KQ;
+        return h->cx ? fkd::find_kset_dc(h->b, *L, *KQ, ks) : fkd::find_kset_dr(h->b, *L, *KQ, ks);
+      }
+  return false;
+}
+template <>
+bool kset_for_width<float>(fkpm_handle* h, int W, fkd::KSet<float>* ks, int* L, int* KQ) {
+  fkd::GeomChoice gs[64];
+  const int ng = fkd::geometries_for_block(h->b, gs, 64);
+  for (int want = 64; want >= 1; want /= 2)
+    for (int k = 0; k < ng; k++)
+      if (gs[k].L == want && gs[k].L * gs[k].KQ == W) {
+        *L = gs[k].L;
+        *KQ = gs[k].KQ;
+        return h->cx ? fkd::find_kset_fc(h->b, *L, *KQ, ks) : fkd::find_kset_fr(h->b, *L, *KQ, ks);
+      }
+  return false;
+}
+
+// C (M x M, FP64, row-major (m1, m2)) = [C +] A^H B for K x M matrices A, B
+// of the storage type (column m at A + m * ld): k_dot_block over slices of
+// 512 k interleaved across about 2048 workgroups, then k_dot_reduce adding
+// the grid rows' partial blocks in order (at most 1 GiB of them).
+template <class S>
+int dot_block_acc(fkpm_handle* h, const S* A, const S* B, long long K, long long ld, int M,
+                  bool first) {
+  const int HS = h->cx ? 2 : 1;
+  const int nbm = (M + fkd::DOT_BM - 1) / fkd::DOT_BM;
+  const int nblk = nbm * nbm;
+  const long long kslice = 512;
+  const size_t cnt = (size_t)M * M * HS;
+  const long long nslices = (K + kslice - 1) / kslice;
+  long long ny = std::max(1, 2048 / nblk);
+  ny = std::min(ny, nslices);
+  ny = std::min<long long>(ny, std::max<long long>(1, ((size_t)1 << 30) / (cnt * sizeof(double))));
+  ny = std::min<long long>(ny, 65535);
+  CHK(h->m2_part.ensure((size_t)ny * cnt * sizeof(double)));
+  double* part = h->m2_part.as<double>();
+  if (h->cx)
+    CHK(launch(&fkd::k_dot_block<S, true>, dim3(nblk, (int)ny), dim3(fkd::WG), h->stream, A, B,
+               ld, 0LL, K, kslice, M, part));
+  else
+    CHK(launch(&fkd::k_dot_block<S, false>, dim3(nblk, (int)ny), dim3(fkd::WG), h->stream, A, B,
+               ld, 0LL, K, kslice, M, part));
+  return launch(&fkd::k_dot_reduce, dim3(grid_for(cnt)), dim3(fkd::WG), h->stream,
+                (const double*)part, (int)ny, cnt, h->m2_c.as<double>(), first ? 1 : 0);
+}
+
+template <class TH, class S, bool CX>
+int upload_current(fkpm_handle* h, int which, int nnzb, const int* rp, const int* ci,
+                   const void* val) {
+  const int nb = h->nb, b = h->b;
+  std::vector<int> rb(nnzb > 0 ? nnzb : 1);
+  for (int i = 0; i < nb; i++)
+    for (int p = rp[i]; p < rp[i + 1]; p++) rb[p] = i;
+  DevBuf &drp = h->j_rp[which], &dci = h->j_ci[which], &drb = h->j_rb[which];
+  DevBuf &draw = h->j_raw[which], &dval = h->j_val[which];
+  const size_t cnt = (size_t)nnzb * b * b;
+  CHK(drp.ensure(sizeof(int) * (nb + 1)));
+  CHK(dci.ensure(sizeof(int) * (nnzb > 0 ? nnzb : 1)));
+  CHK(drb.ensure(sizeof(int) * rb.size()));
+  CHK(draw.ensure(cnt * sizeof(TH) * (CX ? 2 : 1) + 16));
+  CHK(dval.ensure(cnt * sizeof(S) * (CX ? 2 : 1) + 16));
+  HIPCHK(hipMemcpyAsync(drp.p, rp, sizeof(int) * (nb + 1), hipMemcpyHostToDevice, h->stream));
+  if (nnzb > 0) {
+    HIPCHK(hipMemcpyAsync(dci.p, ci, sizeof(int) * nnzb, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(draw.p, val, cnt * sizeof(TH) * (CX ? 2 : 1), hipMemcpyHostToDevice,
+                          h->stream));
+  }
+  HIPCHK(hipMemcpyAsync(drb.p, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice,
+                        h->stream));
+  if (cnt > 0)
+    CHK(launch(&fkd::k_scale_H<TH, S, CX>, dim3(grid_for(cnt)), dim3(256), h->stream,
+               draw.as<const TH>(), dval.as<S>(), drb.as<const int>(), dci.as<const int>(), nnzb,
+               b, 1.0, 0.0));
+  HIPCHK(hipStreamSynchronize(h->stream));  // host arrays (rb, caller's) go out of scope
+  return 0;
+}
+
+template <class S>
+int upload_current_s(fkpm_handle* h, int which, int nnzb, const int* rp, const int* ci,
+                     const void* val) {
+  switch (h->dtype) {
+    case FKPM_F32: return upload_current<float, S, false>(h, which, nnzb, rp, ci, val);
+    case FKPM_F64: return upload_current<double, S, false>(h, which, nnzb, rp, ci, val);
+    case FKPM_C32: return upload_current<float, S, true>(h, which, nnzb, rp, ci, val);
+    default: return upload_current<double, S, true>(h, which, nnzb, rp, ci, val);
+  }
+}
+
+template <class TH, class S, bool CX>
+int pack_R_cols(fkpm_handle* h, int k0, int sc, int spc, S* dst) {
+  const TH* src = h->rraw.as<const TH>() + (size_t)k0 * h->n * (CX ? 2 : 1);
+  const dim3 grid((h->n + 63) / 64, (spc + 15) / 16);
+  return launch(&fkd::k_pack_R<TH, S, CX>, grid, dim3(fkd::WG), h->stream, src, dst, h->n, sc, spc);
+}
+
+template <class S>
+int pack_R_cols_s(fkpm_handle* h, int k0, int sc, int spc, S* dst) {
+  switch (h->dtype) {
+    case FKPM_F32: return pack_R_cols<float, S, false>(h, k0, sc, spc, dst);
+    case FKPM_F64: return pack_R_cols<double, S, false>(h, k0, sc, spc, dst);
+    case FKPM_C32: return pack_R_cols<float, S, true>(h, k0, sc, spc, dst);
+    default: return pack_R_cols<double, S, true>(h, k0, sc, spc, dst);
+  }
+}
+
+template <class S>
+int moments2_t(fkpm_handle* h, int M, int col_chunk, size_t budget, double* mu) {
+  const int HS = h->cx ? 2 : 1;
+  const int ne = h->ne();
+  const size_t es = sizeof(S);
+  // columns per chunk: the requested count (or all), halved until 2M + 3
+  // vectors fit the budget
+  auto padded = [&](int sc, int* W) {
+    const int need = (sc + ne - 1) / ne;
+    fkd::GeomChoice gs[64];
+    const int ng = fkd::geometries_for_block(h->b, gs, 64);
+    int best = 0, widest = 0;
+    for (int k = 0; k < ng; k++) {
+      const int w = gs[k].L * gs[k].KQ;
+      widest = std::max(widest, w);
+      if (w >= need && (best == 0 || w < best)) best = w;
+    }
+    if (best == 0) best = widest;
+    *W = best;
+    return ((need + best - 1) / best) * best * ne;  // padded columns
+  };
+  int sc = col_chunk > 0 ? std::min(col_chunk, h->s) : h->s;
+  int W = 0, spc = padded(sc, &W);
+  auto vec_bytes = [&](int spc_) { return (size_t)h->n * spc_ * HS * es; };
+  while (sc > 1 && (size_t)(2 * M + 3) * vec_bytes(spc) > budget) {
+    sc = (sc + 1) / 2;
+    spc = padded(sc, &W);
+  }
+  if ((size_t)(2 * M + 3) * vec_bytes(spc) > budget)
+    return fail(FKPM_E_ARG, "moments2: M = %d vectors of %zu bytes exceed the %zu-byte budget", M,
+                vec_bytes(spc), budget);
+  fkd::KSet<S> ks;
+  int L = 0, KQ = 0;
+  if (!kset_for_width<S>(h, W, &ks, &L, &KQ))
+    return fail(FKPM_E_UNSUP, "moments2: no kernel of width %d for block %d", W, h->b);
+  const int qpc = spc / ne;
+  KCfg c;
+  c.L = L;
+  c.KQ = KQ;
+  c.maxwg = 4096;
+  derive_grid(h->nb, c);
+  const size_t vb = vec_bytes(spc);
+  // the columns of A and B are spaced by a vector plus 4352 bytes: with K a
+  // power of two a plain spacing would put the same k of every column in the
+  // same HBM channel
+  const size_t vpad = 4352;
+  CHK(h->m2_a.ensure((size_t)M * (vb + vpad)));
+  CHK(h->m2_b.ensure((size_t)M * (vb + vpad)));
+  CHK(h->m2_ring.ensure(3 * vb));
+  CHK(h->m2_c.ensure((size_t)M * M * HS * sizeof(double)));
+  const size_t vlen = vb / es;            // scalars per vector
+  const size_t vs = (vb + vpad) / es;     // scalars between columns of A, B
+  S* A = h->m2_a.as<S>();
+  S* Bm = h->m2_b.as<S>();
+  S* ring = h->m2_ring.as<S>();
+  const int* rp = h->rp.as<const int>();
+  const int* ci = h->ci.as<const int>();
+  const S* hv = h->hv.as<const S>();
+  const dim3 grid(c.nwg), block(fkd::WG);
+  auto step = [&](const int* rp_, const int* ci_, const S* hv_, const S* x, const S* z, S* y,
+                  double cx, double cz) {
+    return launch(ks.mom, grid, block, h->stream, rp_, ci_, hv_, x, z, y, h->nb, qpc, c.chunk,
+                  cx, cz, (double*)nullptr, 0, no_rsource<S>());
+  };
+  const int* j1rp = h->j_rp[0].as<const int>();
+  const int* j1ci = h->j_ci[0].as<const int>();
+  const S* j1v = h->j_val[0].as<const S>();
+  const int* j2rp = h->j_rp[1].as<const int>();
+  const int* j2ci = h->j_ci[1].as<const int>();
+  const S* j2v = h->j_val[1].as<const S>();
+  bool first = true;
+  for (int k0 = 0; k0 < h->s; k0 += sc) {
+    const int scc = std::min(sc, h->s - k0);
+    CHK(pack_R_cols_s<S>(h, k0, scc, spc, A));
+    // alpha_m = T_m(Hs) r into column m of A
+    for (int m = 1; m < M; m++)
+      CHK(step(rp, ci, hv, A + (m - 1) * vs, m >= 2 ? A + (m - 2) * vs : nullptr, A + m * vs,
+               m == 1 ? 1.0 : 2.0, m == 1 ? 0.0 : -1.0));
+    // atild_m = T_m(Hs) j1 r in a ring of three; column m of B = j2 atild_m
+    CHK(step(j1rp, j1ci, j1v, A, nullptr, ring, 1.0, 0.0));
+    for (int m = 0; m < M; m++) {
+      S* t = ring + (m % 3) * vlen;
+      if (m >= 1)
+        CHK(step(rp, ci, hv, ring + ((m - 1) % 3) * vlen, m >= 2 ? ring + ((m - 2) % 3) * vlen : nullptr,
+                 t, m == 1 ? 1.0 : 2.0, m == 1 ? 0.0 : -1.0));
+      CHK(step(j2rp, j2ci, j2v, t, nullptr, Bm + m * vs, 1.0, 0.0));
+    }
+    CHK(dot_block_acc<S>(h, A, Bm, (long long)h->n * spc, (long long)(vs / HS), M, first));
+    first = false;
+  }
+  std::vector<double> cm((size_t)M * M * HS);
+  HIPCHK(hipMemcpyAsync(cm.data(), h->m2_c.p, cm.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (size_t x = 0; x < (size_t)M * M; x++) {
+    mu[2 * x] = cm[x * HS];
+    mu[2 * x + 1] = HS == 2 ? cm[x * HS + 1] : 0.0;
+  }
+  return 0;  // the buffers of moments / autodiff_matrix are untouched
+}
+
 template <class TH, class S, bool CX>
 int upload_H_t(fkpm_handle* h, DevBuf& dst, const void* val, double mag, double shift) {
   const size_t cnt = (size_t)h->nnzb * h->b * h->b;
@@ -1274,6 +1514,33 @@ int fkpm_autodiff_matrix(fkpm_handle* h, int M, const double* c, void* D_val) {
   if (D_val) return copy_D_out(h, D_val);
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
+}
+
+int fkpm_moments2(fkpm_handle* h, int M, int nnzb1, const int* rp1, const int* ci1,
+                  const void* v1, int nnzb2, const int* rp2, const int* ci2, const void* v2,
+                  int col_chunk, double* mu) {
+  if (!h || !mu || !rp1 || !ci1 || !v1 || !rp2 || !ci2 || !v2)
+    return fail(FKPM_E_ARG, "moments2: null argument");
+  if (M < 2 || M % 2 != 0) return fail(FKPM_E_ARG, "moments2: M = %d must be even and >= 2", M);
+  HIPCHK(hipSetDevice(h->device));
+  CHK(ensure_ready(h));
+  CHK(check_pattern("j1", h->nb, nnzb1, rp1, nullptr, ci1));
+  CHK(check_pattern("j2", h->nb, nnzb2, rp2, nullptr, ci2));
+  if (h->single) {
+    CHK(upload_current_s<float>(h, 0, nnzb1, rp1, ci1, v1));
+    CHK(upload_current_s<float>(h, 1, nnzb2, rp2, ci2, v2));
+  } else {
+    CHK(upload_current_s<double>(h, 0, nnzb1, rp1, ci1, v1));
+    CHK(upload_current_s<double>(h, 1, nnzb2, rp2, ci2, v2));
+  }
+  // vector budget: half the free device memory, at most 64 GiB, unless the
+  // environment names one
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  size_t budget = std::min((size_t)64 << 30, free_b / 2);
+  if (const char* e = getenv("FKPM_MOMENTS2_BYTES")) budget = (size_t)strtoull(e, nullptr, 10);
+  return h->single ? moments2_t<float>(h, M, col_chunk, budget, mu)
+                   : moments2_t<double>(h, M, col_chunk, budget, mu);
 }
 
 int fkpm_get_D(fkpm_handle* h, void* D_val) {

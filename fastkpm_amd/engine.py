@@ -265,6 +265,22 @@ class HipEngine:
     def set_autotune_on_use(self, on: bool) -> None:
         call("fkpm_set_option", self._h, _capi.OPT_AUTOTUNE, 1 if on else 0)
 
+    def moments2_v1(self, M: int, j1: "SpMatBsr", j2: "SpMatBsr", col_chunk: int = 0) -> np.ndarray:
+        """Two-current moments mu[m1][m2] = sum_k <T_m1(Hs) r_k, j2 T_m2(Hs) j1 r_k>
+        (Engine::moments2_v1): M x M complex.  j1, j2 on H's block rows, the
+        engine's element type; col_chunk bounds the probe columns per pass."""
+        out = np.zeros(M * M * 2)
+        args = []
+        keep = []
+        for j in (j1, j2):
+            rp = np.ascontiguousarray(j.row_ptr, dtype=np.int32)
+            ci = np.ascontiguousarray(j.col_idx, dtype=np.int32)
+            v = np.ascontiguousarray(j.val, dtype=self.np_dtype)
+            keep += [rp, ci, v]
+            args += [j.n_blocks(), _iptr(rp), _iptr(ci), v.ctypes.data_as(C.c_void_p)]
+        call("fkpm_moments2", self._h, int(M), *args, int(col_chunk), _dptr(out))
+        return out.view(np.complex128).reshape(M, M)
+
     def set_mfma(self, on: bool) -> None:
         """Matrix-core kernels where the block shape has them (complex 8x8
         blocks; on by default), or the vector-unit kernels only."""

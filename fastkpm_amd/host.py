@@ -16,6 +16,7 @@ _lib = None
 _I, _D, _P, _U32 = C.c_int, C.c_double, C.c_void_p, C.c_uint32
 _IP, _DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
 SIGNATURES = {
+    "fkpm_host_last_error": (C.c_char_p, []),
     "fkpm_host_rng_new": (_P, [_U32]),
     "fkpm_host_rng_free": (None, [_P]),
     "fkpm_host_rng_next": (_U32, [_P]),
@@ -25,6 +26,8 @@ SIGNATURES = {
     "fkpm_host_set_R_correlated": (_I, [_I, _I, _IP, _P, _I, _I, _P, _IP]),
     "fkpm_host_set_R_probing": (_I, [_I, _IP, _P, _IP, _P, _IP]),
     "fkpm_host_expansion_coefficients": (_I, [_I, _I, _I, _D, _D, _D, _D, _DP]),
+    "fkpm_host_conductivity_coefficients": (_I, [_I, _I, _D, _D, _D, _D, _D, _DP, _DP]),
+    "fkpm_host_moment_transform2": (_I, [_I, _DP, _I, _DP, _DP]),
     "fkpm_host_jackson_kernel": (None, [_I, _DP]),
     "fkpm_host_moment_product": (_D, [_I, _DP, _DP]),
     "fkpm_host_moment_transform": (None, [_I, _DP, _I, _DP]),
@@ -80,7 +83,8 @@ class RNG:
 
 def _check(rc, what):
     if rc != 0:
-        raise ValueError(f"{what} failed ({rc})")
+        msg = lib().fkpm_host_last_error()
+        raise ValueError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
 
 def set_R_identity(dtype: str, n: int, j0: int = 0, j1: int | None = None) -> np.ndarray:
@@ -174,3 +178,28 @@ def filling_to_mu(gamma, lo, hi, kT, filling, delta_filling=0.0) -> float:
     g = np.ascontiguousarray(gamma, dtype=np.float64)
     return float(lib().fkpm_host_filling_to_mu(len(g), g.ctypes.data_as(_DP), lo, hi, kT,
                                                filling, delta_filling))
+
+
+def conductivity_coefficients(M: int, Mq: int, kT: float, mu: float, omega: float, lo: float,
+                              hi: float, kernel=None) -> np.ndarray:
+    """electrical_conductivity_coefficients(M, Mq, kT, mu, omega, es, kernel)
+    (Jackson kernel by default): M x M complex, unit e^2/h."""
+    k = np.ascontiguousarray(jackson_kernel(M) if kernel is None else kernel, dtype=np.float64)
+    out = np.zeros(M * M, dtype=np.complex128)
+    _check(lib().fkpm_host_conductivity_coefficients(M, Mq, kT, mu, omega, lo, hi,
+                                                     k.ctypes.data_as(_DP),
+                                                     out.ctypes.data_as(_DP)),
+           "conductivity_coefficients")
+    return out.reshape(M, M)
+
+
+def moment_transform2(mu2, Mq: int, kernel=None) -> np.ndarray:
+    """moment_transform of M x M complex moments with a kernel (Jackson by
+    default): Mq x Mq complex."""
+    m = np.ascontiguousarray(mu2, dtype=np.complex128)
+    M = m.shape[0]
+    k = np.ascontiguousarray(jackson_kernel(M) if kernel is None else kernel, dtype=np.float64)
+    out = np.zeros(Mq * Mq, dtype=np.complex128)
+    _check(lib().fkpm_host_moment_transform2(M, m.ctypes.data_as(_DP), Mq, k.ctypes.data_as(_DP),
+                                             out.ctypes.data_as(_DP)), "moment_transform2")
+    return out.reshape(Mq, Mq)

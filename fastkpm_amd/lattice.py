@@ -107,6 +107,28 @@ def lattice(shape, b: int = 1, t: float = -1.0, J: float = 0.0, W: float = 0.0,
     return H
 
 
+def current(H: SpMatBsr, shape, axis: int, real: bool = False) -> SpMatBsr:
+    """Current operator j = i [H, X_axis] of a periodic lattice Hamiltonian
+    (block rows = sites, index x + Lx*y + ...): block (i, j) is
+    i (x_j - x_i) H_ij with the minimum-image displacement; blocks with no
+    displacement (on-site, bonds along other axes) are left out of the
+    pattern.  real=True drops the factor i (a real antisymmetric operator of
+    the same pattern, for real engines)."""
+    xs = _coords(shape)[axis].astype(np.int64)
+    L = shape[axis]
+    b = H.b_len
+    rows = np.repeat(np.arange(H.n_rows), np.diff(np.asarray(H.row_ptr)))
+    cols = np.asarray(H.col_idx, dtype=np.int64)
+    d = (xs[cols] - xs[rows] + L // 2) % L - L // 2
+    keep = d != 0
+    # column-major blocks: element (bi, bj) at bj*b + bi
+    blk = np.transpose(np.asarray(H.val).reshape(-1, b, b), (0, 2, 1))[keep]
+    fac = d[keep].astype(np.float64) if real else 1j * d[keep]
+    out = coo_to_bsr(H.n_rows, b, rows[keep], cols[keep], fac[:, None, None] * blk)
+    out.val = out.val.real.astype(H.val.dtype) if real else out.val.astype(H.val.dtype)
+    return out
+
+
 def colouring(shape, periods, b: int = 1) -> np.ndarray:
     """Probing colours: orbital + b * spatial colour, the spatial colour being
     the site position modulo `periods` (e.g. (4, 4) on a square lattice)."""

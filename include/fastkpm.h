@@ -153,10 +153,20 @@ Vec<double> jackson_kernel(int M);
 Vec<double> lorentz_kernel(int M, double lambda);
 void chebyshev_fill_array(double x, Vec<double>& ret, int kind = 1);
 Vec<double> expansion_coefficients(int M, int Mq, std::function<double(double)> f, EnergyScale es);
+// C_{m1,m2} for the optical (omega > 0) or static (omega = 0) conductivity,
+// unit e^2/h (src/fastkpm.cpp:80-233; the direct-sum form, no FFTW here)
+Vec<Vec<cx_double>> electrical_conductivity_coefficients(int M, int Mq, double kT, double mu,
+                                                         double omega, EnergyScale es,
+                                                         Vec<double> const& kernel);
 double moment_product(Vec<double> const& c, Vec<double> const& mu);
+cx_double moment_product(Vec<Vec<cx_double>> const& c, Vec<Vec<cx_double>> const& mu);
 Vec<double> moment_transform(Vec<double> const& moments, int Mq);
+Vec<Vec<cx_double>> moment_transform(Vec<Vec<cx_double>> const& moments, int Mq,
+                                     Vec<double> const& kernel);
 double density_product(Vec<double> const& gamma, std::function<double(double)> f, EnergyScale es);
 void density_function(Vec<double> const& gamma, EnergyScale es, Vec<double>& x, Vec<double>& rho);
+void density_function(Vec<Vec<cx_double>> const& gamma, EnergyScale es, Vec<double>& x,
+                      Vec<double>& y, Vec<Vec<cx_double>>& rho);
 void integrated_density_function(Vec<double> const& gamma, EnergyScale es, Vec<double>& x,
                                  Vec<double>& irho);
 double fermi_energy(double x, double kT, double mu);
@@ -209,9 +219,10 @@ class Engine {
   // Chebyshev moments: mu_m = tr T_m(Hs) ~ tr R^\dagger T_m(Hs) R
   virtual Vec<double> moments(int M) = 0;
 
-  // Chebyshev moments: mu_{m1,m2} = tr( j1 T_{m1}(Hs) j2 T_{m2}(Hs) ) -- not
-  // provided by this engine (two-current conductivity moments); exits with a
-  // message, as the reference GPU moments2_v2 does.
+  // Chebyshev moments: mu_{m1,m2} = tr( j1 T_{m1}(Hs) j2 T_{m2}(Hs) ).  The HIP
+  // engine computes moments2_v1 on the device (R_chunk_ncols bounds the probe
+  // columns per pass, a_chunk_ncols is unused: all M orders stay in HBM);
+  // moments2_v2 exits with a message, as the reference GPU engine does.
   virtual Vec<Vec<cx_double>> moments2_v1(int M, SpMatBsr<T> const& j1op,
                                           SpMatBsr<T> const& j2op, int a_chunk_ncols = 0,
                                           int R_chunk_ncols = 0) = 0;

@@ -117,6 +117,20 @@ int fkpm_stoch_matrix(fkpm_handle* h, int M, const double* c, void* D_val);
  * Engine_cuSPARSE::autodiff_matrix (engine_cusparse.cpp:955-1039). */
 int fkpm_autodiff_matrix(fkpm_handle* h, int M, const double* c, void* D_val);
 
+/* Two-current moments mu[m1][m2] = sum_k <T_m1(Hs) r_k, j2 T_m2(Hs) j1 r_k>
+ * (M even), written row-major as M*M complex doubles (re, im), for the
+ * current operators j1, j2 given as BSR matrices on H's block rows (nb block
+ * rows of b x b blocks of the engine type, any pattern).  The probe columns
+ * are processed in chunks of at most col_chunk (0: all), halved until the
+ * 2M + 3 vectors of a chunk fit the device-memory budget (half the free
+ * memory, at most 64 GiB, or the byte count in the environment variable
+ * FKPM_MOMENTS2_BYTES); FKPM_E_ARG if one column does not fit.  Replaces
+ * Engine_cuSPARSE::moments2_v1 (engine_cusparse.cpp:793-902).  The state left
+ * by fkpm_moments (for fkpm_autodiff_matrix) is kept. */
+int fkpm_moments2(fkpm_handle* h, int M, int nnzb1, const int* rp1, const int* ci1,
+                  const void* v1, int nnzb2, const int* rp2, const int* ci2, const void* v2,
+                  int col_chunk, double* mu);
+
 /* Copy the last device-resident D to the host (engine type). */
 int fkpm_get_D(fkpm_handle* h, void* D_val);
 

@@ -14,6 +14,10 @@
 extern "C" {
 #endif
 
+/* Message of the last failed fkpm_host_* call on this thread (FKPM_E_ARG
+   return): the text of the exception the C++ host library threw. */
+const char* fkpm_host_last_error(void);
+
 /* std::mt19937 (fkpm::RNG) owned by the library. */
 void* fkpm_host_rng_new(uint32_t seed);
 void fkpm_host_rng_free(void* rng);
@@ -42,6 +46,14 @@ int fkpm_host_set_R_probing(int n, const int* groups, void* rng, int* rcol_out, 
 int fkpm_host_expansion_coefficients(int M, int Mq, int kind, double kT, double mu, double lo,
                                      double hi, double* c_out);
 void fkpm_host_jackson_kernel(int M, double* g_out);
+/* electrical_conductivity_coefficients(M, Mq, kT, mu, omega, es, kernel):
+ * M x M complex written row-major as (re, im) pairs. */
+int fkpm_host_conductivity_coefficients(int M, int Mq, double kT, double mu, double omega,
+                                        double lo, double hi, const double* kernel, double* out);
+/* moment_transform of M x M complex moments (row-major (re, im)) with a
+ * kernel: Mq x Mq complex out. */
+int fkpm_host_moment_transform2(int M, const double* mu, int Mq, const double* kernel,
+                                double* gamma_out);
 double fkpm_host_moment_product(int M, const double* c, const double* mu);
 void fkpm_host_moment_transform(int M, const double* mu, int Mq, double* gamma_out);
 double fkpm_host_electronic_grand_energy(int Mq, const double* gamma, double lo, double hi,

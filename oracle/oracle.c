@@ -547,6 +547,67 @@ int orc_engine_moments_cols(orc_engine* e, int M, double* mu, int c0, int c1) {
 
 /* Engine_CPU::outer_product (engine_cpu.cpp:213-226):
  * D_k(bi,bj) += alpha * sum_l A[b i + bi, l] conj(B[b j + bj, l]). */
+/* y = A x for one column (length n = nb * b) of a BSR matrix A with column-
+ * major blocks (SpMatBsr layout, fastkpm.h:64-106). */
+static void orc_bsr_apply_col(int nb, int b, const int* rp, const int* ci, const cplx* val,
+                              const cplx* x, cplx* y) {
+    int bb = b * b;
+    for (int i = 0; i < nb; i++)
+        for (int bi = 0; bi < b; bi++) {
+            cplx acc = 0;
+            for (int p = rp[i]; p < rp[i + 1]; p++) {
+                int j = ci[p];
+                for (int bj = 0; bj < b; bj++) acc += val[(size_t)p * bb + bj * b + bi] * x[(size_t)b * j + bj];
+            }
+            y[(size_t)b * i + bi] = acc;
+        }
+}
+
+/* Engine_CPU::moments2_v1 (engine_cpu.cpp:86-160): for every probe column r_k,
+ * alpha_m = T_m(Hs) r_k and atild_m = T_m(Hs) j1 r_k (m < M), and
+ * mu[m1][m2] += cdot(alpha_m1, j2 atild_m2) = sum_i conj(alpha_m1[i]) (j2 atild_m2)[i].
+ * j1, j2 are BSR on H's block rows, unscaled.  mu is M x M row-major.  The
+ * reference's column chunking (a_chunk_ncols) only bounds its memory; the sum
+ * is the same. */
+int orc_engine_moments2_v1(orc_engine* e, int M, int nnzb1, const int* rp1, const int* ci1,
+                           const cplx* v1, int nnzb2, const int* rp2, const int* ci2, const cplx* v2,
+                           cplx* mu) {
+    (void)nnzb1; (void)nnzb2;
+    if (M < 2 || M % 2 != 0 || !e->R) return -1;
+    int n = e->n, b = e->b, nb = e->nb, s = e->s;
+    cplx* al = (cplx*)malloc(sizeof(cplx) * (size_t)M * n);
+    cplx* at = (cplx*)malloc(sizeof(cplx) * (size_t)M * n);
+    cplx* bt = (cplx*)malloc(sizeof(cplx) * (size_t)M * n);
+    cplx* tmp = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    for (size_t q = 0; q < (size_t)M * M; q++) mu[q] = 0;
+    for (int k = 0; k < s; k++) {
+        for (int i = 0; i < n; i++) al[i] = e->R[(size_t)i * s + k];
+        orc_bsr_apply_col(nb, b, rp1, ci1, v1, al, at); /* atild_0 = j1 r */
+        for (int m = 1; m < M; m++) {
+            cplx* am = al + (size_t)m * n;
+            cplx* tm = at + (size_t)m * n;
+            orc_bsr_apply_col(nb, b, e->row_ptr, e->col_idx, e->hs, al + (size_t)(m - 1) * n, am);
+            orc_bsr_apply_col(nb, b, e->row_ptr, e->col_idx, e->hs, at + (size_t)(m - 1) * n, tm);
+            if (m >= 2)
+                for (int i = 0; i < n; i++) {
+                    am[i] = 2.0 * am[i] - al[(size_t)(m - 2) * n + i];
+                    tm[i] = 2.0 * tm[i] - at[(size_t)(m - 2) * n + i];
+                }
+        }
+        for (int m = 0; m < M; m++) orc_bsr_apply_col(nb, b, rp2, ci2, v2, at + (size_t)m * n, bt + (size_t)m * n);
+        for (int m1 = 0; m1 < M; m1++)
+            for (int m2 = 0; m2 < M; m2++) {
+                cplx acc = 0;
+                const cplx* x = al + (size_t)m1 * n;
+                const cplx* y = bt + (size_t)m2 * n;
+                for (int i = 0; i < n; i++) acc += conj(x[i]) * y[i];
+                mu[(size_t)m1 * M + m2] += acc;
+            }
+    }
+    free(al); free(at); free(bt); free(tmp);
+    return 0;
+}
+
 static void orc_outer_product(const orc_engine* e, double alpha, const cplx* A, const cplx* B, int nnzbD,
                               const int* D_row_idx, const int* D_col_idx, cplx* D_val) {
     int b = e->b, bb = b * b, s = e->s;

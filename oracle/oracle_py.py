@@ -62,6 +62,7 @@ def lib():
             "orc_engine_moments_cols": (_I, [_P, _I, _DP, _I, _I]),
             "orc_engine_stoch_matrix": (_I, [_P, _I, _DP, _I, _IP, _IP, _P]),
             "orc_engine_autodiff_matrix": (_I, [_P, _I, _DP, _I, _I, _IP, _IP, _IP, _P]),
+            "orc_engine_moments2_v1": (_I, [_P, _I, _I, _IP, _IP, _P, _I, _IP, _IP, _P, _P]),
             "orc_energy_scale": (_I, [_I, _I, _I, _IP, _IP, _P, _D, _I, _P, _DP, _DP, _DP, _DP]),
         }
         for name, (res, args) in sig.items():
@@ -288,6 +289,19 @@ class Engine:
         lib().orc_engine_stoch_matrix(self._e, len(c), _d(c), D.n_blocks(), _i(_i32(D.row_idx)),
                                       _i(_i32(D.col_idx)), _p(out))
         return out
+
+    def moments2_v1(self, M, j1, j2):
+        """Engine_CPU::moments2_v1 (engine_cpu.cpp:86-160): M x M complex."""
+        mu = np.zeros(M * M, np.complex128)
+        v1 = np.ascontiguousarray(j1.val, dtype=np.complex128)
+        v2 = np.ascontiguousarray(j2.val, dtype=np.complex128)
+        rc = lib().orc_engine_moments2_v1(self._e, M, j1.n_blocks(), _i(_i32(j1.row_ptr)),
+                                          _i(_i32(j1.col_idx)), _p(v1), j2.n_blocks(),
+                                          _i(_i32(j2.row_ptr)), _i(_i32(j2.col_idx)), _p(v2),
+                                          _p(mu))
+        if rc != 0:
+            raise ValueError(f"oracle moments2_v1 failed ({rc})")
+        return mu.reshape(M, M)
 
     def autodiff_matrix(self, c, D):
         c = np.ascontiguousarray(c, dtype=np.float64)

@@ -1,6 +1,7 @@
 // Drives the fkpm:: C++ API the way an application of the reference does
 // (build H with SpMatElems, energy_scale, set_R_correlated, set_H, moments,
-// expansion_coefficients, moment_product, autodiff_matrix, stoch_matrix).
+// expansion_coefficients, moment_product, autodiff_matrix, stoch_matrix,
+// moments2_v1, electrical_conductivity_coefficients).
 //
 //   engine_flow host          host-only API checks (no device needed)
 //   engine_flow device DIR    full flow on HIP device 0; writes H, R, mu and
@@ -137,10 +138,39 @@ static int device_flow(const std::string& dir) {
   eng->autodiff_matrix(cE, D);
   SpMatBsr<cx_double> D2 = H;
   eng->stoch_matrix(cF, D2);
+  // two-current moments with the x current j_x = i [H, X] (hopping bonds
+  // along x only), probe columns in chunks of 5, then the static
+  // conductivity sigma_xx = sum c[m1][m2] mu2[m1][m2]
+  SpMatElems<cx_double> jel(L * L, L * L, 2);
+  for (int y = 0; y < L; y++)
+    for (int x = 0; x < L; x++) {
+      const int i = x + L * y;
+      const cx_double fw[4] = {cx_double(0, -1.0), 0, 0, cx_double(0, -1.0)};  // i t (+1), t = -1
+      const cx_double bw[4] = {cx_double(0, 1.0), 0, 0, cx_double(0, 1.0)};    // i t (-1)
+      jel.add(i, ((x + 1) % L) + L * y, fw);
+      jel.add(i, ((x + L - 1) % L) + L * y, bw);
+    }
+  const SpMatBsr<cx_double> jx(jel);
+  // stoch_matrix consumed the moments state; take it again before moments2
+  const auto mu_again = eng->moments(M);
+  CHECK(mu_again == mu);
+  const int M2 = 16;
+  const auto mu2 = eng->moments2_v1(M2, jx, jx, 10, 5);
+  CHECK((int)mu2.size() == M2 && (int)mu2[0].size() == M2);
+  const auto c2 = electrical_conductivity_coefficients(M2, 2 * M2, 0.05, 0.0, 0.0, es, jackson_kernel(M2));
+  const cx_double sigma = moment_product(c2, mu2);
+  // moments2 leaves the moments state alone: autodiff with the same M still
+  // works and repeats the first result bit for bit
+  SpMatBsr<cx_double> D3 = H;
+  eng->autodiff_matrix(cE, D3);
+  CHECK(D3.val == D.val);
+  Vec<cx_double> mu2_flat;
+  for (auto const& row : mu2) mu2_flat.insert(mu2_flat.end(), row.begin(), row.end());
   std::ofstream meta(dir + "/meta.txt");
   meta.precision(17);
   meta << n << " " << eng->R.n_cols << " " << M << " " << H.n_rows << " " << H.n_blocks() << " "
-       << es.lo << " " << es.hi << " " << E << "\n";
+       << es.lo << " " << es.hi << " " << E << " " << M2 << " " << sigma.real() << " "
+       << sigma.imag() << " " << jx.n_blocks() << "\n";
   dump(dir + "/H_row_ptr.bin", H.row_ptr.data(), H.row_ptr.size());
   dump(dir + "/H_row_idx.bin", H.row_idx.data(), H.row_idx.size());
   dump(dir + "/H_col_idx.bin", H.col_idx.data(), H.col_idx.size());
@@ -151,6 +181,11 @@ static int device_flow(const std::string& dir) {
   dump(dir + "/cF.bin", cF.data(), cF.size());
   dump(dir + "/D_autodiff.bin", D.val.data(), D.val.size());
   dump(dir + "/D_stoch.bin", D2.val.data(), D2.val.size());
+  dump(dir + "/mu2.bin", mu2_flat.data(), mu2_flat.size());
+  dump(dir + "/j_row_ptr.bin", jx.row_ptr.data(), jx.row_ptr.size());
+  dump(dir + "/j_row_idx.bin", jx.row_idx.data(), jx.row_idx.size());
+  dump(dir + "/j_col_idx.bin", jx.col_idx.data(), jx.col_idx.size());
+  dump(dir + "/j_val.bin", jx.val.data(), jx.val.size());
   std::printf("device OK E=%.12g\n", E);
   return 0;
 }

@@ -35,6 +35,8 @@ def test_cpp_api_device_flow_matches_oracle(oracle, tmp_path):
     meta = (tmp_path / "meta.txt").read_text().split()
     n, s, M, nb, nnzb = (int(v) for v in meta[:5])
     lo, hi, E = (float(v) for v in meta[5:8])
+    M2 = int(meta[8])
+    sigma = complex(float(meta[9]), float(meta[10]))
     rd = lambda name, dt: np.fromfile(tmp_path / name, dtype=dt)
     H = SpMatBsr(nb, nb, 2, rd("H_row_idx.bin", np.int32), rd("H_col_idx.bin", np.int32),
                  rd("H_row_ptr.bin", np.int32), rd("H_val.bin", np.complex128))
@@ -55,3 +57,15 @@ def test_cpp_api_device_flow_matches_oracle(oracle, tmp_path):
     # restatement of engine.cpp draws the same phases
     groups = np.argmax(np.abs(R) > 0, axis=1)
     assert np.array_equal(oracle.set_R_correlated(groups, oracle.MT19937(2024)), R)
+    # two-current moments (x current, probe columns in chunks of 5) and the
+    # static conductivity from them
+    J = SpMatBsr(nb, nb, 2, rd("j_row_idx.bin", np.int32), rd("j_col_idx.bin", np.int32),
+                 rd("j_row_ptr.bin", np.int32), rd("j_val.bin", np.complex128))
+    assert J.n_blocks() == int(meta[11])
+    mu2_o = orc.moments2_v1(M2, J, J)
+    mu2 = rd("mu2.bin", np.complex128).reshape(M2, M2)
+    assert rel_err(mu2, mu2_o) < 1e-10
+    from fastkpm_amd import host
+    c2 = host.conductivity_coefficients(M2, 2 * M2, 0.05, 0.0, 0.0, lo, hi)
+    s_o = complex(np.sum(c2 * mu2_o))
+    assert abs(sigma - s_o) < 1e-10 * abs(s_o)

@@ -88,3 +88,104 @@ def test_filling_to_mu_inverts_filling(host, oracle):
         m = host.filling_to_mu(gamma, lo, hi, kT, filling)
         occ = sum(g * oracle.fermi_density(x, kT, m) for g, x in zip(gamma, xs)) / tot
         assert occ == pytest.approx(filling, abs=1e-6)
+
+
+def _dct_form_static(oracle, M, Mq, kT, mu, lo, hi, kernel):
+    """Static conductivity coefficients from DCT-II / DST-II transforms of two
+    node functions (the FFTW branch of src/fastkpm.cpp:90-140): an algebraically
+    different route from the library's direct node sum."""
+    from scipy.fft import dct, dst
+    avg, mag = (hi + lo) / 2, (hi - lo) / 2
+    x = np.cos(np.pi * (np.arange(Mq) + 0.5) / Mq)
+    t = 1.0 - x * x
+    f = np.array([oracle.fermi_density(xi * mag + avg, kT, mu) for xi in x])
+    keep = t >= 1e-4
+    x1 = np.where(keep, f * x / np.where(keep, t, 1.0) ** 1.5, 0.0)
+    x2 = np.where(keep, f / np.where(keep, t, 1.0), 0.0)
+    yc1, yc2, ys1, ys2 = dct(x1, 2), dct(x2, 2), dst(x1, 2), dst(x2, 2)
+
+    def c_(y, k):
+        return y[abs(k)]
+
+    def s_(y, k):
+        return 0.0 if k == 0 else np.sign(k) * y[abs(k) - 1]
+
+    out = np.zeros((M, M), dtype=np.complex128)
+    w = np.where(np.arange(M) == 0, 1.0, 2.0) * kernel
+    for a in range(M):
+        for b in range(M):
+            ms, md = a + b, a - b
+            yr = 2 * yc1[ms] + 2 * c_(yc1, md) + (ms * ys2[ms - 1] if ms > 0 else 0.0) \
+                + md * s_(ys2, md)
+            yi = 2 * s_(ys1, md) - md * (yc2[ms] + c_(yc2, md))
+            out[a, b] = np.pi * w[a] * w[b] / (2 * Mq * mag * mag) * (yr + 1j * yi)
+    return out
+
+
+def _dct_form_optical(oracle, M, Mq, kT, mu, omega, lo, hi, kernel):
+    """Optical conductivity coefficients, one DCT-II per m1 (the FFTW branch of
+    src/fastkpm.cpp:176-205)."""
+    from scipy.fft import dct
+    avg, mag = (hi + lo) / 2, (hi - lo) / 2
+    ws = omega / mag
+    i0 = int(np.ceil(np.arccos(1.0 - ws) / np.pi * Mq - 0.5))
+    x = np.cos(np.pi * (np.arange(Mq) + 0.5) / Mq)
+    out = np.zeros((M, M), dtype=np.complex128)
+    for a in range(M):
+        X = np.zeros(Mq)
+        for i in range(i0, Mq):
+            xw = x[i] + ws
+            if x[i] < -1 + 1e-4 or xw > 1 - 1e-4:
+                continue
+            e = x[i] * mag + avg
+            df = oracle.fermi_density(e, kT, mu) - oracle.fermi_density(e + omega, kT, mu)
+            X[i] = 0.5 * np.cos(a * np.arccos(xw)) * df / (ws * np.sqrt(1 - xw * xw))
+        out[a, :] = dct(X, 2)[:M]
+    w = np.where(np.arange(M) == 0, 1.0, 2.0) * kernel
+    return out * (2 * np.pi * np.outer(w, w) / (Mq * mag * mag))
+
+
+@pytest.mark.parametrize("kT,mu", [(0.05, 0.0), (0.2, -0.7), (0.0, 0.3)])
+def test_conductivity_static_matches_dct_form(host, oracle, kT, mu):
+    M, Mq, lo, hi = 24, 96, -3.2, 2.6
+    g = host.jackson_kernel(M)
+    got = host.conductivity_coefficients(M, Mq, kT, mu, 0.0, lo, hi)
+    want = _dct_form_static(oracle, M, Mq, kT, mu, lo, hi, g)
+    assert np.abs(got - want).max() < 1e-11 * np.abs(want).max()
+    assert np.abs(want).max() > 0
+
+
+@pytest.mark.parametrize("omega", [0.3, 1.1])
+def test_conductivity_optical_matches_dct_form(host, oracle, omega):
+    M, Mq, lo, hi, kT, mu = 20, 200, -3.0, 3.0, 0.1, 0.2
+    g = host.jackson_kernel(M)
+    got = host.conductivity_coefficients(M, Mq, kT, mu, omega, lo, hi)
+    want = _dct_form_optical(oracle, M, Mq, kT, mu, omega, lo, hi, g)
+    assert np.abs(got - want).max() < 1e-11 * np.abs(want).max()
+    assert np.abs(got.imag).max() == 0.0
+
+
+def test_conductivity_edges(host):
+    M = 8
+    # omega at or beyond the scaled bandwidth: all zero
+    assert not np.any(host.conductivity_coefficients(M, 32, 0.1, 0.0, 4.0, -1.0, 1.0))
+    with pytest.raises(ValueError, match="Mq"):
+        host.conductivity_coefficients(M, 2 * M - 1, 0.1, 0.0, 0.0, -1.0, 1.0)
+    with pytest.raises(ValueError, match="omega"):
+        host.conductivity_coefficients(M, 32, 0.1, 0.0, -0.1, -1.0, 1.0)
+    with pytest.raises(ValueError, match="20 nodes"):
+        host.conductivity_coefficients(M, 32, 0.1, 0.0, 1.9, -1.0, 1.0)
+
+
+def test_moment_transform2_is_separable(host):
+    """For moments a (x) b the 2-D transform is the outer product of the 1-D
+    transforms (both apply the Jackson kernel)."""
+    rng = np.random.default_rng(4)
+    M, Mq = 14, 40
+    a = rng.normal(size=M) + 1j * rng.normal(size=M)
+    b = rng.normal(size=M) + 1j * rng.normal(size=M)
+    got = host.moment_transform2(np.outer(a, b), Mq)
+    ta = host.moment_transform(a.real, Mq) + 1j * host.moment_transform(a.imag, Mq)
+    tb = host.moment_transform(b.real, Mq) + 1j * host.moment_transform(b.imag, Mq)
+    want = np.outer(ta, tb)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()

@@ -361,3 +361,63 @@ def test_tridiagonal_extremes_and_lanczos(oracle):
     assert lo == pytest.approx(lam[0], abs=1e-8) and hi == pytest.approx(lam[-1], abs=1e-8)
     lo2, hi2, _, _ = oracle.energy_scale(H, 0.1, n, v)
     assert lo2 == pytest.approx(lo - 0.1 * (hi - lo)) and hi2 == pytest.approx(hi + 0.1 * (hi - lo))
+
+
+def test_moments2_identity_currents_closed_form(oracle):
+    """j1 = j2 = I and R = I: mu2[m1][m2] = tr T_m1 T_m2 = (mu_{m1+m2} + mu_{|m1-m2|}) / 2."""
+    from fastkpm_amd import lattice
+    H = lattice.lattice((3, 3), b=2, J=1.3, seed=2)
+    n = H.n_rows * 2
+    e = oracle.Engine()
+    e.set_H(H, -6.0, 6.0)
+    e.set_R(np.eye(n, dtype=complex))
+    M = 10
+    I = lattice.coo_to_bsr(H.n_rows, 2, list(range(H.n_rows)), list(range(H.n_rows)),
+                           np.array([np.eye(2)] * H.n_rows))
+    mu2 = e.moments2_v1(M, I, I)
+    mu = e.moments(2 * M)
+    ref = np.array([[0.5 * (mu[a + b] + mu[abs(a - b)]) for b in range(M)] for a in range(M)])
+    assert np.max(np.abs(mu2 - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_moments2_matches_dense_traces(oracle):
+    """mu2[m1][m2] = tr(R^+ T_m1(Hs)^+ j2 T_m2(Hs) j1 R) by dense matrix algebra."""
+    from fastkpm_amd import lattice
+    H = lattice.lattice((3, 3), b=2, J=1.3, seed=2)
+    n = H.n_rows * 2
+    lo, hi = -6.0, 6.0
+    Hd = H.to_dense()
+    Hs = (Hd - (hi + lo) / 2 * np.eye(n)) / ((hi - lo) / 2)
+    rs = np.random.default_rng(1)
+    J1, J2 = H.like(), H.like()
+    J1.val = rs.normal(size=H.val.shape) + 1j * rs.normal(size=H.val.shape)
+    J2.val = rs.normal(size=H.val.shape) + 0j
+    M = 10
+    T = [np.eye(n), Hs]
+    for _ in range(2, M):
+        T.append(2 * Hs @ T[-1] - T[-2])
+    R = rs.normal(size=(n, 3)) + 1j * rs.normal(size=(n, 3))
+    e = oracle.Engine()
+    e.set_H(H, lo, hi)
+    e.set_R(R)
+    mu2 = e.moments2_v1(M, J1, J2)
+    ref = np.array([[np.trace(R.conj().T @ T[a].conj().T @ J2.to_dense() @ T[b] @ J1.to_dense() @ R)
+                     for b in range(M)] for a in range(M)])
+    assert np.max(np.abs(mu2 - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_lattice_current_is_commutator():
+    """lattice.current(H, shape, axis) is Hermitian and, on a lattice too
+    long to wrap within one hop's minimum image, equals i [H, X] with X the
+    position along the axis unwrapped (bonds across the boundary aside)."""
+    from fastkpm_amd import lattice
+    shape = (7, 5)
+    H = lattice.lattice(shape, b=2, J=1.1, W=0.5, seed=4)
+    J = lattice.current(H, shape, 0)
+    Jd, Hd = J.to_dense(), H.to_dense()
+    assert np.allclose(Jd, Jd.conj().T, atol=1e-14)
+    x = np.repeat(np.arange(np.prod(shape)) % shape[0], 2).astype(float)
+    comm = 1j * (Hd * x[None, :] - x[:, None] * Hd)
+    cross = np.abs(x[:, None] - x[None, :]) > 1  # bonds across the boundary
+    assert np.allclose(np.where(cross, 0, Jd), np.where(cross, 0, comm), atol=1e-14)
+    assert np.all(np.abs(np.diag(Jd)) == 0)
