@@ -58,6 +58,8 @@ SIGNATURES = {
                                C.POINTER(C.c_longlong)]),
     "fkpm_batch_submit_probing": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _D, _I, _I, _IP, _P,
                                        _I, _DP, C.POINTER(C.c_longlong)]),
+    "fkpm_batch_submit_seeded": (_I, [_P, _I, _I, _I, _IP, _IP, _P, _D, _D, _I, _I, _IP,
+                                      C.POINTER(C.c_uint), _I, _DP, C.POINTER(C.c_longlong)]),
     "fkpm_batch_poll": (_I, [_P, C.c_longlong, _IP]),
     "fkpm_batch_wait": (_I, [_P, C.c_longlong, _DP, _DP, _DP, _DP]),
     "fkpm_batch_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),

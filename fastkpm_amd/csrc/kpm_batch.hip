@@ -210,6 +210,87 @@ __global__ __launch_bounds__(WG) void k_batch_scatter_R(const int* __restrict__ 
   }
 }
 
+// Probing vectors generated on the device (Engine::set_R_correlated,
+// src/engine.cpp:76-89): column j's generator is std::mt19937(seed_j), seed_j
+// the j-th draw of the caller's master engine, and the rows of colour j draw
+// their phases from it in row order through
+// std::uniform_int_distribution<int>(0, 3), which libstdc++ (Lemire's
+// downscaling, no rejection for a range of 4) reduces to the top two bits of
+// the 32-bit draw: phase = (+1, +i, -1, -i)[y >> 30].
+//
+// One workgroup of one wave per column.  The wave walks the rows in chunks of
+// 64; the rows of colour j in a chunk take consecutive outputs of the
+// generator (their rank among the chunk's hits).  The generator state (624
+// words) sits in LDS and is regenerated ("twisted") by the whole wave in four
+// phases whose reads never see a word of the wrong generation:
+//   k in [0, 227):   uses old mt[k], mt[k+1], mt[k+397]
+//   k in [227, 454): uses new mt[k-227], old mt[k], mt[k+1]
+//   k in [454, 623): same, mt[k-227] from the previous phase
+//   k = 623:         uses new mt[396], new mt[0], old mt[623]
+// Every row has exactly one colour in [0, s), so every rval is written once.
+__global__ __launch_bounds__(64) void k_probe_phases(const int* __restrict__ groups, int n,
+                                                     const unsigned* __restrict__ seeds,
+                                                     double* __restrict__ rval) {
+  __shared__ unsigned mt[624];
+  const int j = blockIdx.x, lane = threadIdx.x;
+  if (lane == 0) {
+    unsigned x = seeds[j];
+    mt[0] = x;
+    for (int i = 1; i < 624; i++) {
+      x = 1812433253u * (x ^ (x >> 30)) + (unsigned)i;
+      mt[i] = x;
+    }
+  }
+  __syncthreads();
+  auto twist_range = [&](int k0, int k1) {
+    for (int kb = k0; kb < k1; kb += 64) {
+      const int k = kb + lane;
+      unsigned nv = 0;
+      if (k < k1) {
+        const unsigned y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+        nv = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+      }
+      __syncthreads();
+      if (k < k1) mt[k] = nv;
+      __syncthreads();
+    }
+  };
+  auto twist = [&]() {
+    twist_range(0, 227);
+    twist_range(227, 454);
+    twist_range(454, 623);
+    twist_range(623, 624);
+  };
+  int pos = 624;  // next unused output of the current block (624: none left)
+  for (int base = 0; base < n; base += 64) {
+    const int i = base + lane;
+    const bool hit = i < n && groups[i] == j;
+    const unsigned long long mask = __ballot(hit);
+    const int cnt = __popcll(mask);
+    if (cnt == 0) continue;  // the same for the whole wave
+    const int idx = pos + __popcll(mask & ((1ull << lane) - 1ull));
+    unsigned y = 0;
+    if (hit && idx < 624) y = mt[idx];
+    if (pos + cnt > 624) {
+      __syncthreads();
+      twist();
+      if (hit && idx >= 624) y = mt[idx - 624];
+      pos += cnt - 624;
+    } else {
+      pos += cnt;
+    }
+    if (hit) {
+      y ^= y >> 11;
+      y ^= (y << 7) & 0x9d2c5680u;
+      y ^= (y << 15) & 0xefc60000u;
+      y ^= y >> 18;
+      const unsigned ph = y >> 30;
+      rval[2 * (size_t)i] = ph == 0 ? 1.0 : (ph == 2 ? -1.0 : 0.0);
+      rval[2 * (size_t)i + 1] = ph == 1 ? 1.0 : (ph == 3 ? -1.0 : 0.0);
+    }
+  }
+}
+
 // End of a tick: every active slot moves to its next step.
 __global__ void k_batch_advance(int* __restrict__ st, const int* __restrict__ act, int na) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,7 +388,9 @@ struct Request {
   std::vector<int> rp, ci, rowblk, tpos;
   std::vector<double> val, R, c;
   bool compact = false;  // R given as (rcol, rval) per orbital row (R holds rval)
+  bool seeded = false;   // compact R generated on the device: rcol = colours
   std::vector<int> rcol;
+  std::vector<unsigned> seeds;  // seeded: one generator seed per column
   // results
   std::vector<double> mu, D;
   double energy = 0;
@@ -376,7 +459,8 @@ void plan(bool single, const Request& r, Layout* L) {
   len[B_HV] = (size_t)r.nnzb * r.b * r.b * 2 * es;
   len[B_HRAW] = dbl;
   len[B_R] = len[B_A0] = len[B_A1] = len[B_A2] = len[B_B0] = len[B_B1] = len[B_B2] = vec;
-  len[B_RRAW] = r.compact ? (size_t)r.n * (sizeof(int) + 2 * sizeof(double)) + 256
+  len[B_RRAW] = r.compact ? (size_t)r.n * (sizeof(int) + 2 * sizeof(double)) + 512 +
+                                (r.seeded ? sizeof(unsigned) * r.seeds.size() : 0)
                          : (size_t)r.n * r.s * 2 * sizeof(double);
   len[B_DACC] = len[B_DOUT] = dbl;
   len[B_PART] = h * nwg * 2 * sizeof(double);
@@ -477,7 +561,19 @@ void admit_one(fkpm_batch* B, Slot& sl) {
   BCHK(hipMemcpyAsync(P(B_HRAW), r.val.data(), L.len[B_HRAW], hipMemcpyHostToDevice, st));
   // compact R: rcol then rval (16-byte aligned) in the staging buffer
   const size_t rv_off = ((size_t)r.n * sizeof(int) + 255) & ~(size_t)255;
-  if (r.compact) {
+  if (r.seeded) {
+    // colours and column seeds up; the phases are drawn on the device
+    const size_t sd_off = (rv_off + sizeof(double) * 2 * r.n + 255) & ~(size_t)255;
+    BCHK(hipMemcpyAsync(P(B_RRAW), r.rcol.data(), sizeof(int) * r.n, hipMemcpyHostToDevice, st));
+    BCHK(hipMemcpyAsync((char*)P(B_RRAW) + sd_off, r.seeds.data(), sizeof(unsigned) * r.s,
+                        hipMemcpyHostToDevice, st));
+    const int* g = (const int*)P(B_RRAW);
+    const unsigned* sd = (const unsigned*)((char*)P(B_RRAW) + sd_off);
+    double* rv = (double*)((char*)P(B_RRAW) + rv_off);
+    const int n = r.n;
+    void* argv[] = {(void*)&g, (void*)&n, (void*)&sd, (void*)&rv};
+    BCHK(hipLaunchKernel((const void*)&fkd::k_probe_phases, dim3(r.s), dim3(64), argv, 0, st));
+  } else if (r.compact) {
     BCHK(hipMemcpyAsync(P(B_RRAW), r.rcol.data(), sizeof(int) * r.n, hipMemcpyHostToDevice, st));
     BCHK(hipMemcpyAsync((char*)P(B_RRAW) + rv_off, r.R.data(), sizeof(double) * 2 * r.n,
                         hipMemcpyHostToDevice, st));
@@ -889,8 +985,9 @@ int fkpm_batch_set_option(fkpm_batch* B, int option, int value) {
 static int submit_common(fkpm_batch* B, int nb, int b, int nnzb, const int* row_ptr,
                          const int* col_idx, const void* val, double lo, double hi, int n, int s,
                          const void* R_colmajor, const int* rcol, const void* rval, int M,
-                         const double* c, long long* id) {
-  if (!B || !row_ptr || !col_idx || !val || !(R_colmajor || (rcol && rval)) || !c || !id)
+                         const double* c, long long* id, const unsigned* seeds = nullptr) {
+  if (!B || !row_ptr || !col_idx || !val || !(R_colmajor || (rcol && (rval || seeds))) || !c ||
+      !id)
     return berr(FKPM_E_ARG, "batch_submit: null argument");
   if (b != 2 && b != 8) return berr(FKPM_E_UNSUP, "batch: block size %d (2 or 8 supported)", b);
   if (nb <= 0 || nnzb < nb || n != nb * b || s <= 0 || M < 2 || M % 2 != 0 || !(hi > lo))
@@ -936,6 +1033,14 @@ static int submit_common(fkpm_batch* B, int nb, int b, int nnzb, const int* row_
   r->val.assign((const double*)val, (const double*)val + hvn);
   if (R_colmajor) {
     r->R.assign((const double*)R_colmajor, (const double*)R_colmajor + (size_t)n * s * 2);
+  } else if (seeds) {
+    r->compact = r->seeded = true;
+    r->rcol.assign(rcol, rcol + n);
+    for (int i = 0; i < n; i++)
+      if (r->rcol[i] < 0 || r->rcol[i] >= s)
+        return berr(FKPM_E_ARG, "batch_submit_seeded: colour %d of row %d not in [0, %d)",
+                    r->rcol[i], i, s);
+    r->seeds.assign(seeds, seeds + s);
   } else {
     r->compact = true;
     r->rcol.assign(rcol, rcol + n);
@@ -977,6 +1082,18 @@ int fkpm_batch_submit_probing(fkpm_batch* B, int nb, int b, int nnzb, const int*
   if (!rcol || !rval) return berr(FKPM_E_ARG, "batch_submit_probing: null probing vectors");
   return submit_common(B, nb, b, nnzb, row_ptr, col_idx, val, lo, hi, n, s, nullptr, rcol, rval,
                        M, c, id);
+}
+
+int fkpm_batch_submit_seeded(fkpm_batch* B, int nb, int b, int nnzb, const int* row_ptr,
+                             const int* col_idx, const void* val, double lo, double hi, int n,
+                             int s, const int* groups, const unsigned* col_seeds, int M,
+                             const double* c, long long* id) {
+  if (!groups || !col_seeds) return berr(FKPM_E_ARG, "batch_submit_seeded: null colours/seeds");
+  if (s >= n)
+    return berr(FKPM_E_ARG, "batch_submit_seeded: %d colours for %d rows (R = I: submit it)", s,
+                n);
+  return submit_common(B, nb, b, nnzb, row_ptr, col_idx, val, lo, hi, n, s, nullptr, groups,
+                       nullptr, M, c, id, col_seeds);
 }
 
 int fkpm_batch_poll(fkpm_batch* B, long long id, int* done) {

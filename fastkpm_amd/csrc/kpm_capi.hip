@@ -17,6 +17,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 
@@ -1211,27 +1212,40 @@ int lanczos_t(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const
   if (!ok) return fail(FKPM_E_UNSUP, "no Lanczos kernel for block %d", b);
   const int HS = CX ? 2 : 1;
   const int bb = b * b;
-  // pattern with every diagonal block present
-  std::vector<int> rp2(nb + 1), ci2;
-  std::vector<double> hv2;
-  ci2.reserve((size_t)nnzb + nb);
-  hv2.reserve(((size_t)nnzb + nb) * bb * HS);
-  rp2[0] = 0;
+  // pattern with every diagonal block present; when H has them all (the
+  // usual case) its arrays are uploaded as they are
+  int missing = 0;
   for (int i = 0; i < nb; i++) {
     bool diag = false;
-    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) diag |= (col_idx[p] == i);
-    for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) {
-      ci2.push_back(col_idx[p]);
-      const TH* v = static_cast<const TH*>(val) + (size_t)p * bb * HS;
-      for (int e = 0; e < bb * HS; e++) hv2.push_back((double)v[e]);
-    }
-    if (!diag) {
-      ci2.push_back(i);
-      for (int e = 0; e < bb * HS; e++) hv2.push_back(0.0);
-    }
-    rp2[i + 1] = (int)ci2.size();
+    for (int p = row_ptr[i]; p < row_ptr[i + 1] && !diag; p++) diag = (col_idx[p] == i);
+    missing += diag ? 0 : 1;
   }
-  const int nnz2 = (int)ci2.size();
+  const bool direct = missing == 0 && std::is_same<TH, double>::value;
+  std::vector<int> rp2, ci2;
+  std::vector<double> hv2;
+  if (!direct) {
+    rp2.assign(nb + 1, 0);
+    ci2.reserve((size_t)nnzb + missing);
+    hv2.reserve(((size_t)nnzb + missing) * bb * HS);
+    for (int i = 0; i < nb; i++) {
+      bool diag = false;
+      for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) diag |= (col_idx[p] == i);
+      for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) {
+        ci2.push_back(col_idx[p]);
+        const TH* v = static_cast<const TH*>(val) + (size_t)p * bb * HS;
+        for (int e = 0; e < bb * HS; e++) hv2.push_back((double)v[e]);
+      }
+      if (!diag) {
+        ci2.push_back(i);
+        for (int e = 0; e < bb * HS; e++) hv2.push_back(0.0);
+      }
+      rp2[i + 1] = (int)ci2.size();
+    }
+  }
+  const int* up_rp = direct ? row_ptr : rp2.data();
+  const int* up_ci = direct ? col_idx : ci2.data();
+  const double* up_hv = direct ? static_cast<const double*>(val) : hv2.data();
+  const int nnz2 = direct ? nnzb : (int)ci2.size();
   const int n = nb * b;
   const int qp = geoL * geoKQ;  // one probe column padded to one lane-group pass
   const int sp = qp * (2 / HS);
@@ -1243,23 +1257,27 @@ int lanczos_t(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const
   nwg = (nb + chunk - 1) / chunk;
 
   struct Bufs {
-    DevBuf rp, ci, hv, v0, v1, w, part, ps;
+    DevBuf rp, ci, hv, v0, v1, w, part, ps, coef, used;
     ~Bufs() {
-      DevBuf* all[] = {&rp, &ci, &hv, &v0, &v1, &w, &part, &ps};
+      DevBuf* all[] = {&rp, &ci, &hv, &v0, &v1, &w, &part, &ps, &coef, &used};
       for (DevBuf* d : all) d->release();
     }
   } B;
-  CHK(B.rp.ensure(sizeof(int) * rp2.size()));
-  CHK(B.ci.ensure(sizeof(int) * ci2.size()));
-  CHK(B.hv.ensure(sizeof(double) * hv2.size()));
+  const size_t hv_count = (size_t)nnz2 * bb * HS;
+  CHK(B.rp.ensure(sizeof(int) * (nb + 1)));
+  CHK(B.ci.ensure(sizeof(int) * std::max(nnz2, 1)));
+  CHK(B.hv.ensure(sizeof(double) * std::max<size_t>(hv_count, 1)));
   CHK(B.v0.ensure(len * sizeof(double)));
   CHK(B.v1.ensure(len * sizeof(double)));
   CHK(B.w.ensure(len * sizeof(double)));
   CHK(B.part.ensure((size_t)nwg * 2 * sizeof(double)));
   CHK(B.ps.ensure(2 * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(B.rp.p, rp2.data(), sizeof(int) * rp2.size(), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(B.ci.p, ci2.data(), sizeof(int) * ci2.size(), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(B.hv.p, hv2.data(), sizeof(double) * hv2.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(B.rp.p, up_rp, sizeof(int) * (nb + 1), hipMemcpyHostToDevice, h->stream));
+  if (nnz2 > 0) {
+    HIPCHK(hipMemcpyAsync(B.ci.p, up_ci, sizeof(int) * nnz2, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(B.hv.p, up_hv, sizeof(double) * hv_count, hipMemcpyHostToDevice,
+                          h->stream));
+  }
   CHK(zero_doubles(h, B.v0.p, len));
   CHK(zero_doubles(h, B.w.p, len));
   {
@@ -1278,43 +1296,43 @@ int lanczos_t(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, const
     HIPCHK(hipMemcpyAsync(B.v1.p, packed.data(), len * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));  // host staging vectors go out of scope
   }
+  // the iteration runs on the stream without a host round trip: alpha, beta
+  // and the invariant-subspace cut stay on the device until the end
   const dim3 grid(nwg), block(fkd::WG);
-  double ps[2];
-  auto spmv_dot = [&](double* out_alpha) -> int {
-    // w = H v1, alpha = Re<w, v1>
+  CHK(B.coef.ensure(2 * (size_t)iters * sizeof(double)));
+  CHK(B.used.ensure(sizeof(int)));
+  double* coef = static_cast<double*>(B.coef.p);
+  int* dused = static_cast<int*>(B.used.p);
+  const double* dps = static_cast<const double*>(B.ps.p);
+  auto spmv_dot = [&]() -> int {
+    // w = H v1; ps = (|v1|^2, Re <w, v1>)
     CHK(launch(ks.mom, grid, block, h->stream, (const int*)B.rp.p, (const int*)B.ci.p,
                (const double*)B.hv.p, (const double*)B.v1.p, (const double*)nullptr,
                (double*)B.w.p, nb, qp, chunk, 1.0, 0.0, (double*)B.part.p, 0, no_rsource<double>()));
-    CHK(launch(&fkd::k_reduce_partials, dim3(1), block, h->stream, (const double*)B.part.p, nwg,
-               (double*)B.ps.p));
-    HIPCHK(hipMemcpyAsync(ps, B.ps.p, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    *out_alpha = ps[1];
-    return 0;
+    return launch(&fkd::k_reduce_partials, dim3(1), block, h->stream, (const double*)B.part.p,
+                  nwg, (double*)B.ps.p);
   };
-  std::vector<double> alpha(iters, 0.0), beta(iters, 0.0);
-  int used = iters;
+  CHK(launch(&fkd::k_lanczos_edge, dim3(1), dim3(64), h->stream, dps, coef, iters, dused, 0));
   for (int j = 1; j < iters; j++) {
-    CHK(spmv_dot(&alpha[j - 1]));
-    // w = w - alpha v1 - beta v0; beta_j = |w|
+    CHK(spmv_dot());
     CHK(launch(&fkd::k_lanczos_orth, dim3(nwg), block, h->stream, (double*)B.w.p,
-               (const double*)B.v1.p, (const double*)B.v0.p, len, alpha[j - 1], beta[j - 1],
+               (const double*)B.v1.p, (const double*)B.v0.p, len, dps, coef, iters, j,
                (double*)B.part.p));
     CHK(launch(&fkd::k_reduce_partials, dim3(1), block, h->stream, (const double*)B.part.p, nwg,
                (double*)B.ps.p));
-    HIPCHK(hipMemcpyAsync(ps, B.ps.p, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    beta[j] = std::sqrt(ps[0]);
-    if (!(beta[j] > 1e-300)) {  // invariant subspace: the j x j matrix is exact
-      used = j;
-      break;
-    }
     CHK(launch(&fkd::k_lanczos_next, dim3(grid_for(len)), dim3(256), h->stream, (double*)B.v0.p,
-               (double*)B.v1.p, (const double*)B.w.p, len, 1.0 / beta[j]));
+               (double*)B.v1.p, (const double*)B.w.p, len, dps, coef, iters, j, dused));
   }
-  if (used == iters) CHK(spmv_dot(&alpha[iters - 1]));
-  alpha.resize(used);
-  beta.resize(used);
+  CHK(spmv_dot());
+  CHK(launch(&fkd::k_lanczos_edge, dim3(1), dim3(64), h->stream, dps, coef, iters, dused, 1));
+  std::vector<double> cf(2 * (size_t)iters);
+  int used = iters;
+  HIPCHK(hipMemcpyAsync(cf.data(), coef, cf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipMemcpyAsync(&used, dused, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> alpha(cf.begin(), cf.begin() + used), beta(cf.begin() + iters,
+                                                                 cf.begin() + iters + used);
   double emin = 0, emax = 0;
   tridiag_extremes(alpha, beta, &emin, &emax);
   const double slack = extend * (emax - emin);

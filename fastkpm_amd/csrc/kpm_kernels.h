@@ -1028,14 +1028,24 @@ __global__ void k_symmetrize_scale(const double* __restrict__ din, double* __res
 
 #ifdef FKD_DEFINE_HELPERS
 // ---------------------------------------------------------------------------
-// Lanczos helpers (FP64, one column): w -= a v1 + b v0 with |w|^2 partials;
-// v0 <- v1, v1 <- w / beta.
+// Lanczos helpers (FP64, one column) with the coefficients kept on the
+// device, so the loop runs without a host round trip per iteration.
+// coef = [alpha_0 .. alpha_{it-1}, beta_0 .. beta_{it-1}], beta_0 = 0; ps =
+// the two reduced sums of the previous kernel.  Step j (1 <= j < it):
+//   orth:  alpha_{j-1} = ps[1] = <H v1, v1>;  w -= alpha_{j-1} v1 + beta_{j-1} v0;
+//          partial sums of |w|^2
+//   next:  beta_j = sqrt(ps[0]); v0 = v1; v1 = w / beta_j (0 if beta_j = 0);
+//          used = j at the first beta_j <= 1e-300 (an invariant subspace: the
+//          j x j tridiagonal matrix is exact)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WG) void k_lanczos_orth(double* __restrict__ w,
                                                      const double* __restrict__ v1,
                                                      const double* __restrict__ v0, size_t n,
-                                                     double a, double b,
+                                                     const double* __restrict__ ps,
+                                                     double* __restrict__ coef, int it, int j,
                                                      double* __restrict__ partial) {
+  const double a = ps[1], b = coef[it + j - 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) coef[j - 1] = a;
   double s = 0.0;
   for (size_t k = (size_t)blockIdx.x * WG + threadIdx.x; k < n; k += (size_t)gridDim.x * WG) {
     const double v = w[k] - a * v1[k] - b * v0[k];
@@ -1046,11 +1056,33 @@ __global__ __launch_bounds__(WG) void k_lanczos_orth(double* __restrict__ w,
 }
 
 __global__ void k_lanczos_next(double* __restrict__ v0, double* __restrict__ v1,
-                               const double* __restrict__ w, size_t n, double inv_beta) {
+                               const double* __restrict__ w, size_t n,
+                               const double* __restrict__ ps, double* __restrict__ coef, int it,
+                               int j, int* __restrict__ used) {
+  const double beta = sqrt(ps[0]);
+  const bool alive = beta > 1e-300;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    coef[it + j] = beta;
+    if (!alive && *used == it) *used = j;
+  }
+  const double inv = alive ? 1.0 / beta : 0.0;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x) {
     v0[k] = v1[k];
-    v1[k] = w[k] * inv_beta;
+    v1[k] = w[k] * inv;
+  }
+}
+
+// last = 0: the start of the table (beta_0 = 0, used = it); last = 1: the
+// final alpha_{it-1} = ps[1].
+__global__ void k_lanczos_edge(const double* __restrict__ ps, double* __restrict__ coef, int it,
+                               int* __restrict__ used, int last) {
+  if (threadIdx.x != 0) return;
+  if (last) {
+    coef[it - 1] = ps[1];
+  } else {
+    coef[it] = 0.0;
+    *used = it;
   }
 }
 

@@ -120,6 +120,13 @@ def set_R_probing(groups, rng: RNG):
     return rcol, rval, s.value
 
 
+def master_seeds(rng: RNG, s: int) -> np.ndarray:
+    """The s draws of the caller's engine that set_R_correlated spends on
+    seeding the columns' generators (src/engine.cpp:84-86), in order: the
+    seeds a device-side generation of the same R takes."""
+    return np.array([rng() for _ in range(int(s))], dtype=np.uint32)
+
+
 def set_R_correlated(dtype: str, groups, rng: RNG, j0: int | None = None,
                      j1: int | None = None) -> np.ndarray:
     """Engine<T>::set_R_correlated(groups, rng[, j0, j1])."""

@@ -75,15 +75,26 @@ class BatchEngine:
         except Exception:
             pass
 
-    def submit(self, H: SpMatBsr, es: EnergyScale, c, R=None, probing=None) -> int:
-        """Queue a request; R dense (n x s) or probing = (rcol, rval, s)."""
+    def submit(self, H: SpMatBsr, es: EnergyScale, c, R=None, probing=None,
+               seeded=None) -> int:
+        """Queue a request; R dense (n x s), probing = (rcol, rval, s) or
+        seeded = (groups, seeds): colours per row and one generator seed per
+        column (master_seeds), the phases then drawn on the device."""
         rp = np.ascontiguousarray(H.row_ptr, dtype=np.int32)
         ci = np.ascontiguousarray(H.col_idx, dtype=np.int32)
         val = np.ascontiguousarray(H.val, dtype=np.complex128)
         c = np.ascontiguousarray(c, dtype=np.float64)
         n = H.n_rows * H.b_len
         rid = C.c_longlong()
-        if probing is not None:
+        if seeded is not None:
+            groups, seeds = seeded
+            groups = np.ascontiguousarray(groups, dtype=np.int32)
+            seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+            call("fkpm_batch_submit_seeded", self._h, H.n_rows, H.b_len, H.n_blocks(), _ip(rp),
+                 _ip(ci), val.ctypes.data_as(C.c_void_p), float(es.lo), float(es.hi), n,
+                 len(seeds), _ip(groups), seeds.ctypes.data_as(C.POINTER(C.c_uint)), len(c),
+                 _dp(c), C.byref(rid))
+        elif probing is not None:
             rcol, rval, s = probing
             rcol = np.ascontiguousarray(rcol, dtype=np.int32)
             rval = np.ascontiguousarray(rval, dtype=np.complex128)
@@ -194,7 +205,8 @@ def draw_spec(rng: np.random.Generator, n_max: int = 1 << 20, m_max: int = 2048)
 
 
 def build_request(spec: RequestSpec, kT: float = 0.05, mu: float = 0.0):
-    """H, energy scale, coefficients and compact probing vectors of a spec."""
+    """H, energy scale, coefficients and the probing vectors of a spec as
+    colours plus column seeds (drawn on the device at admission)."""
     if spec.b == 2:
         H = lattice.lattice(spec.shape, b=2, t=-1.0, J=1.0, W=spec.W, seed=spec.seed)
     else:
@@ -209,8 +221,8 @@ def build_request(spec: RequestSpec, kT: float = 0.05, mu: float = 0.0):
     es = gershgorin(H)
     c = host.expansion_coefficients(spec.M, 2 * spec.M, host.FERMI_ENERGY, kT, mu, es.lo, es.hi)
     groups = lattice.colouring(spec.shape, (spec.period,) * spec.dims, spec.b)
-    probing = host.set_R_probing(groups, host.RNG(spec.seed))
-    return H, es, c, probing
+    seeded = (groups, host.master_seeds(host.RNG(spec.seed), int(groups.max()) + 1))
+    return H, es, c, seeded
 
 
 def run_stream(eng: BatchEngine, specs, rate_hz: float, seed: int = 0, want_D: bool = False,
@@ -224,11 +236,11 @@ def run_stream(eng: BatchEngine, specs, rate_hz: float, seed: int = 0, want_D: b
     t0 = time.perf_counter()
     arrival = t0 + np.cumsum(gaps) - gaps[0]
     ids = []
-    for k, (spec, (H, es, c, probing)) in enumerate(zip(specs, built)):
+    for k, (spec, (H, es, c, seeded)) in enumerate(zip(specs, built)):
         now = time.perf_counter()
         if arrival[k] > now:
             time.sleep(arrival[k] - now)
-        ids.append(eng.submit(H, es, c, probing=probing))
+        ids.append(eng.submit(H, es, c, seeded=seeded))
         if log:
             log(f"submitted {k}: N={spec.N} b={spec.b} S={spec.S} M={spec.M}")
     out = []

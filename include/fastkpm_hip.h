@@ -238,6 +238,17 @@ int fkpm_batch_submit_probing(fkpm_batch* b, int nb, int blk, int nnzb, const in
                               int s, const int* rcol, const void* rval, int M, const double* c,
                               long long* id);
 
+/* The same with the probing vectors drawn on the device: groups[i] in
+ * [0, s) is the colour of orbital row i (s < n) and col_seeds[j] the seed of
+ * column j's std::mt19937, i.e. the j-th draw of the caller's master engine;
+ * R is then bit-identical to Engine::set_R_correlated(groups, rng)
+ * (src/engine.cpp:76-89, complex phases).  Only n colours and s seeds cross
+ * PCIe. */
+int fkpm_batch_submit_seeded(fkpm_batch* b, int nb, int blk, int nnzb, const int* row_ptr,
+                             const int* col_idx, const void* val, double lo, double hi, int n,
+                             int s, const int* groups, const unsigned* col_seeds, int M,
+                             const double* c, long long* id);
+
 /* *done = 1 once the request has finished (or failed). */
 int fkpm_batch_poll(fkpm_batch* b, long long id, int* done);
 

@@ -135,6 +135,33 @@ def test_batch_probing_form_equals_dense(oracle):
     eng.close()
 
 
+@pytest.mark.parametrize("shape,b,per", [((64, 64), 2, (2, 2)), ((5, 5, 4), 8, (1, 1, 2)),
+                                         ((9, 7), 2, (3, 7))])
+def test_batch_seeded_equals_host_probing(shape, b, per):
+    """Probing vectors drawn on the device from the column seeds equal the
+    host's set_R_correlated bit for bit: the results match exactly.  The
+    first case gives every colour 1024 rows (generator blocks of 624 words
+    are crossed inside a 64-row chunk)."""
+    from fastkpm_amd import host, lattice
+    from fastkpm_amd.serving import BatchEngine, gershgorin
+    if b == 2:
+        H = lattice.lattice(shape, b=2, J=1.2, W=1.0, seed=31)
+    else:
+        H = lattice.lattice(shape, b=8, orbital_blocks=True, seed=31)
+    es = gershgorin(H)
+    M = 24
+    c = host.expansion_coefficients(M, 2 * M, host.FERMI_ENERGY, 0.05, 0.0, es.lo, es.hi)
+    groups = lattice.colouring(shape, per, b)
+    s = int(groups.max()) + 1
+    probing = host.set_R_probing(groups, host.RNG(2024))
+    seeds = host.master_seeds(host.RNG(2024), s)
+    eng = BatchEngine("fp64", hbm_bytes=1 << 28, max_slots=4)
+    a = eng.wait(eng.submit(H, es, c, probing=probing))
+    d = eng.wait(eng.submit(H, es, c, seeded=(groups, seeds)))
+    assert np.array_equal(a.mu, d.mu) and np.array_equal(a.D, d.D)
+    eng.close()
+
+
 def test_batch_refusals(oracle):
     from fastkpm_amd import lattice, EnergyScale
     from fastkpm_amd._capi import FkpmError
@@ -152,4 +179,12 @@ def test_batch_refusals(oracle):
     P = lattice.coo_to_bsr(2, 2, [0, 1], [1, 0], np.ones((2, 2, 2)))
     with pytest.raises(FkpmError, match="diagonal"):
         eng.submit(P, EnergyScale(-3, 3), np.ones(4), R=np.ones((4, 1), dtype=np.complex128))
+    # seeded: a colour outside [0, s), and as many colours as rows
+    H2 = lattice.lattice((4, 4), b=2, seed=3)
+    g = lattice.colouring((4, 4), (2, 2), 2)
+    with pytest.raises(FkpmError, match="colour"):
+        eng.submit(H2, EnergyScale(-6, 6), np.ones(4), seeded=(g, np.arange(3, dtype=np.uint32)))
+    with pytest.raises(FkpmError, match="colours"):
+        eng.submit(H2, EnergyScale(-6, 6), np.ones(4),
+                   seeded=(np.arange(32), np.arange(32, dtype=np.uint32)))
     eng.close()

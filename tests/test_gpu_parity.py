@@ -178,6 +178,50 @@ def test_energy_scale_matches_oracle(oracle):
     assert es.lo <= lam[0] and es.hi >= lam[-1]
 
 
+def _ring_no_diagonal(nb=6):
+    """A ring of 2x2 hopping blocks with no on-site blocks (the Lanczos
+    pattern then gains explicit zero diagonal blocks)."""
+    from fastkpm_amd import lattice
+    rows, cols, blks = [], [], []
+    T = -np.array([[1.0, 0.5j], [-0.5j, 2.0]])
+    for i in range(nb):
+        j = (i + 1) % nb
+        rows += [i, j]
+        cols += [j, i]
+        blks += [T, T.conj().T]
+    return lattice.coo_to_bsr(nb, 2, rows, cols, np.array(blks))
+
+
+@pytest.mark.parametrize("case", ["invariant_subspace", "no_diagonal", "real", "cx_float"])
+def test_energy_scale_cases(oracle, case):
+    """Lanczos on the device, coefficients kept there until the end: an
+    iteration count beyond n (the recursion meets an invariant subspace and
+    stops at the exact extremes), H without diagonal blocks, real and single
+    precision engines."""
+    from fastkpm_amd import HipEngine, lattice
+    if case == "no_diagonal":
+        H, dt, iters = _ring_no_diagonal(), "cx_double", 40
+    elif case == "real":
+        H, dt, iters = lattice.lattice((6, 5), b=1, W=1.0, seed=3, dtype=np.float64), "double", 20
+    elif case == "cx_float":
+        H, dt, iters = lattice.lattice((4, 4), b=2, J=1.0, seed=3), "cx_float", 24
+    else:
+        H, dt, iters = lattice.lattice((3, 3), b=2, J=1.5, seed=4), "cx_double", 40
+    n = H.n_rows * H.b_len
+    rs = np.random.default_rng(7)
+    v = rs.standard_normal(n)
+    if dt != "double":
+        v = v + 1j * rs.standard_normal(n)
+    eng = HipEngine(dt, "fp64")
+    es = eng.energy_scale(H, 0.0, iters, v_start=v)
+    lo, hi, _, _ = oracle.energy_scale(H, 0.0, iters, v)
+    tol = 1e-5 if dt == "cx_float" else 1e-9
+    assert abs(es.lo - lo) < tol * abs(lo) and abs(es.hi - hi) < tol * abs(hi)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    if iters >= n:
+        assert abs(es.lo - lam[0]) < 1e-9 * abs(lam[0]) and abs(es.hi - lam[-1]) < 1e-9 * abs(lam[-1])
+
+
 def test_autodiff_requires_moments(oracle):
     from fastkpm_amd import lattice, EnergyScale
     from fastkpm_amd._capi import FkpmError

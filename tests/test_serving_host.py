@@ -42,6 +42,25 @@ def test_stream_specs_stay_in_range():
 def test_build_request_small():
     rng = np.random.default_rng(3)
     sp = draw_spec(rng, n_max=5000, m_max=256)
-    H, es, c, (rcol, rval, s) = build_request(sp)
-    assert H.n_rows * H.b_len == sp.N and len(c) == sp.M and s == sp.S
-    assert es.lo < es.hi and len(rcol) == sp.N
+    H, es, c, (groups, seeds) = build_request(sp)
+    assert H.n_rows * H.b_len == sp.N and len(c) == sp.M and len(seeds) == sp.S
+    assert es.lo < es.hi and len(groups) == sp.N
+    assert groups.min() == 0 and groups.max() == sp.S - 1
+
+
+def test_master_seeds_are_the_column_seeds():
+    """set_R_probing seeds column j's generator with the j-th master draw:
+    the seeds of the device-side generation are those draws, and the master
+    engine ends in the same state either way."""
+    from fastkpm_amd import host, lattice
+    groups = lattice.colouring((6, 4), (3, 2), 2)
+    s = int(groups.max()) + 1
+    a, b = host.RNG(41), host.RNG(41)
+    seeds = host.master_seeds(a, s)
+    rcol, rval, s2 = host.set_R_probing(groups, b)
+    assert s2 == s and a() == b()
+    # column 0's phases from mt19937(seeds[0]): uniform_int(0, 3) is the top two bits
+    g = host.RNG(int(seeds[0]))
+    rows = np.flatnonzero(groups == 0)
+    phases = np.array([1, 1j, -1, -1j])[[g() >> 30 for _ in rows]]
+    assert np.array_equal(rval[rows], phases) and np.all(rcol[rows] == 0)
