@@ -372,6 +372,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-conductivity", action="store_true",
                     help="skip the two-current (conductivity) moments")
     ap.add_argument("--conductivity-moments", type=int, default=128)
+    ap.add_argument("--conductivity-only", action="store_true",
+                    help="run only the two-current moments leg and print it as one JSON line")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving stream, at the full request ranges")
     ap.add_argument("--serve-requests", type=int, default=16)
@@ -398,6 +400,23 @@ def main(argv=None) -> int:
                           "value": sv["fp64"]["updates_per_s"], "unit": "updates/s",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
                           "data": "synthetic", "serving": sv}), flush=True)
+        return 0
+
+    if args.conductivity_only:
+        H, groups, cfg = build_workload(args.config, args.seed)
+        eng = HipEngine("cx_double", "fp64")
+        es = EnergyScale(*cfg["es"]) if "es" in cfg else eng.energy_scale(H, 0.01)
+        eng.close()
+        _log(f"two-current moments, M={args.conductivity_moments}")
+        cv = conductivity_leg(H, groups, cfg, es, args.seed, args.conductivity_moments)
+        N = H.n_rows * H.b_len
+        print(json.dumps({"metric": "two-current (conductivity) moments moments2_v1: ms per call",
+                          "value": cv["fp64"]["ms"], "unit": "ms", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": f"{args.config}: "
+                                                 f"{WORKLOADS.get(args.config, args.config)}",
+                                     "N": N, "M": args.conductivity_moments},
+                          "conductivity": cv}), flush=True)
         return 0
 
     # the practical streaming ceiling on this device: a plain 1 GiB copy
