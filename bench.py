@@ -365,7 +365,7 @@ def main(argv=None) -> int:
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-sample-moments", type=int, default=24)
+    ap.add_argument("--cpu-sample-moments", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-mixed comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")

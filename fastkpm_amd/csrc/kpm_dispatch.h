@@ -69,12 +69,13 @@ bool find_kset_fr(int B, int L, int KQ, KSet<float>* out);   // float, real
 // The list of compiled (B, L, KQ) geometries.  B = 1, 2 take up to 4 packets
 // per lane; B = 4 up to 2; B = 8 one (its accumulators are 8x larger).  One
 // packet per lane keeps the register count (and so the occupancy) of the
-// fused autodiff step low at a given pass width.
-#define FKD_FOR_EACH_GEOMETRY(X)                                                              \
-  X(1, 4, 1) X(1, 4, 2) X(1, 8, 2) X(1, 16, 1) X(1, 16, 2) X(1, 16, 4) X(1, 32, 1) X(1, 32, 4) \
-  X(1, 64, 1) X(1, 64, 4)                                                                     \
-  X(2, 4, 1) X(2, 4, 2) X(2, 8, 2) X(2, 16, 1) X(2, 16, 2) X(2, 16, 4) X(2, 32, 1) X(2, 32, 4) \
-  X(2, 64, 1) X(2, 64, 4)                                                                     \
-  X(4, 4, 1) X(4, 4, 2) X(4, 8, 2) X(4, 16, 1) X(4, 16, 2) X(4, 32, 1) X(4, 32, 2) X(4, 64, 1) \
-  X(4, 64, 2)                                                                                 \
+// fused autodiff step low at a given pass width; four packets over eight
+// lanes gives it longer loads per lane (the FP32 autodiff step at S = 64).
+#define FKD_FOR_EACH_GEOMETRY(X)                                                       \
+  X(1, 4, 1) X(1, 4, 2) X(1, 8, 2) X(1, 8, 4) X(1, 16, 1) X(1, 16, 2) X(1, 16, 4)      \
+  X(1, 32, 1) X(1, 32, 4) X(1, 64, 1) X(1, 64, 4)                                      \
+  X(2, 4, 1) X(2, 4, 2) X(2, 8, 2) X(2, 8, 4) X(2, 16, 1) X(2, 16, 2) X(2, 16, 4)      \
+  X(2, 32, 1) X(2, 32, 4) X(2, 64, 1) X(2, 64, 4)                                      \
+  X(4, 4, 1) X(4, 4, 2) X(4, 8, 2) X(4, 16, 1) X(4, 16, 2) X(4, 32, 1) X(4, 32, 2)     \
+  X(4, 64, 1) X(4, 64, 2)                                                              \
   X(8, 4, 1) X(8, 8, 1) X(8, 16, 1) X(8, 32, 1) X(8, 64, 1)
