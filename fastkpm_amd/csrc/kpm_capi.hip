@@ -152,6 +152,7 @@ struct KCfg {
   int L = 0, KQ = 0, maxwg = 0, nt = 0;
   int mfma = 0;  // matrix-core kernel (one wave per block row: L = 64, KQ = 1):
                  // 1 general, 2 row-resident (rows of <= 8 blocks, vectors < 4 GiB)
+  int sweep = 0;  // rows: 0 per-XCD regions, 1 one front over all rows (chunk = -1)
   int nwg = 0, chunk = 0;
 };
 
@@ -199,6 +200,7 @@ struct fkpm_handle {
   int stoch_method = FKPM_STOCH_CLENSHAW;
   DevBuf R, A[3], Bv[3], xi, partial, psum, tune_dacc;
   int alpha_M = 0;  // M of the moments whose final state is held, 0 if none
+  int sweep_force = -1;  // FKPM_OPT_SWEEP: row sweep of every kind (-1: per configuration)
 
   // --- D ---
   int nbD = 0, nnzbD = 0;
@@ -273,7 +275,7 @@ void derive_grid(int nb, KCfg& c) {
   int chunk = (nb + nwg - 1) / nwg;
   chunk = ((chunk + gwg - 1) / gwg) * gwg;
   c.nwg = (nb + chunk - 1) / chunk;
-  c.chunk = chunk;
+  c.chunk = c.sweep ? -1 : chunk;
 }
 
 // Matrix-core kernels exist for complex 8x8 blocks (forward/Clenshaw steps
@@ -298,6 +300,7 @@ int set_kind(fkpm_handle* h, int kind, KCfg c) {
   }
   if (!lookup_kset(h, kind, h->b, c.L, c.KQ))
     return fail(FKPM_E_UNSUP, "no kernel for block %d, lanes %d, packets %d", h->b, c.L, c.KQ);
+  if (h->sweep_force >= 0) c.sweep = h->sweep_force;
   derive_grid(h->nb, c);
   h->kc[kind] = c;
   return 0;
@@ -565,23 +568,27 @@ int tune_t(fkpm_handle* h) {
       const int w = gs[g].L * gs[g].KQ;
       if (w > h->wmax || 2 * w < h->wmax) continue;
       for (int cap : caps)
-        for (int nt : nts) {
-          KCfg c;
-          c.L = gs[g].L;
-          c.KQ = gs[g].KQ;
-          c.maxwg = cap;
-          c.nt = nt;
-          cands.push_back(c);
-        }
+        for (int nt : nts)
+          for (int sw = 0; sw < 2; sw++) {
+            KCfg c;
+            c.L = gs[g].L;
+            c.KQ = gs[g].KQ;
+            c.maxwg = cap;
+            c.nt = nt;
+            c.sweep = sw;
+            cands.push_back(c);
+          }
     }
     if (mfma_avail(h, kind))
       for (int v = 1; v <= (mfma_rr_avail(h, kind) ? 2 : 1); v++)
-        for (int cap : caps) {
-          KCfg c;
-          c.mfma = v;
-          c.maxwg = cap;
-          cands.push_back(c);
-        }
+        for (int cap : caps)
+          for (int sw = 0; sw < 2; sw++) {
+            KCfg c;
+            c.mfma = v;
+            c.maxwg = cap;
+            c.sweep = sw;
+            cands.push_back(c);
+          }
     {
       for (const KCfg& c : cands) {
           CHK(set_kind(h, kind, c));
@@ -618,8 +625,9 @@ int tune_t(fkpm_handle* h) {
           float ms = 0.f;
           HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
           if (log)
-            fprintf(stderr, "[fkpm tune] kind=%d L=%d KQ=%d mfma=%d maxwg=%d nwg=%d nt=%d: %.2f us\n",
-                    kind, k.L, k.KQ, k.mfma, k.maxwg, k.nwg, k.nt, 1e3 * ms / 6);
+            fprintf(stderr,
+                    "[fkpm tune] kind=%d L=%d KQ=%d mfma=%d maxwg=%d nwg=%d nt=%d sweep=%d: %.2f us\n",
+                    kind, k.L, k.KQ, k.mfma, k.maxwg, k.nwg, k.nt, k.sweep, 1e3 * ms / 6);
           if (ms < best_ms) {
             best_ms = ms;
             best = k;
@@ -1632,6 +1640,13 @@ int fkpm_get_kernel_mfma(fkpm_handle* h, int kind, int* mfma) {
   return 0;
 }
 
+int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep) {
+  if (!h || !sweep) return fail(FKPM_E_ARG, "null argument");
+  if (kind < 0 || kind >= NKIND) return fail(FKPM_E_ARG, "bad kernel kind %d", kind);
+  *sweep = h->kc[kind].sweep;
+  return 0;
+}
+
 int fkpm_get_state(fkpm_handle* h, int* r_compact, int* tuned, int* moments_M) {
   if (!h) return fail(FKPM_E_ARG, "null handle");
   if (r_compact) *r_compact = h->r_compact ? 1 : 0;
@@ -1673,6 +1688,16 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
       return 0;
     case FKPM_OPT_AUTOTUNE:
       h->autotune_on_use = value != 0;
+      return 0;
+    case FKPM_OPT_SWEEP:
+      if (value < -1 || value > 1) return fail(FKPM_E_ARG, "bad sweep %d", value);
+      h->sweep_force = value;
+      if (value >= 0)
+        for (int k = 0; k < NKIND; k++)
+          if (h->kc[k].L > 0) {
+            h->kc[k].sweep = value;
+            derive_grid(h->nb, h->kc[k]);
+          }
       return 0;
     case FKPM_OPT_MFMA:
       if (value != 0 && value != 1) return fail(FKPM_E_ARG, "bad MFMA switch %d", value);

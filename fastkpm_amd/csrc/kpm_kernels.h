@@ -286,8 +286,22 @@ struct WgId {
 
 __device__ __forceinline__ WgId wg_self() { return WgId{(int)blockIdx.x, (int)gridDim.x}; }
 
+//
+// chunk < 0 selects the other sweep: all workgroups advance one front over
+// all rows (workgroup k takes passes k, k + n, ...).  A neighbour d rows away
+// is then d / step workgroups later, on the same XCD when that is a multiple
+// of 8, and the rows in flight across the chip span a narrow window whose
+// far neighbours (a 3-D lattice's next plane) stay in the shared last-level
+// cache rather than eight separate windows.
 __device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step, WgId wid) {
   const int k = wid.k, n = wid.n;
+  if (chunk < 0) {
+    RowRange rr;
+    rr.r0 = k * step;
+    rr.r1 = nb;
+    rr.stride = n * step;
+    return rr;
+  }
   const int x = k & 7, r = k >> 3;
   const int per = n >> 3, extra = n & 7;
   const int nx = per + (x < extra ? 1 : 0);        // workgroups of XCD x

@@ -281,6 +281,11 @@ class HipEngine:
         call("fkpm_moments2", self._h, int(M), *args, int(col_chunk), _dptr(out))
         return out.view(np.complex128).reshape(M, M)
 
+    def set_sweep(self, sweep: int) -> None:
+        """Row sweep of every kernel kind: 0 per XCD, 1 one front, -1 the
+        configuration's own (FKPM_OPT_SWEEP)."""
+        call("fkpm_set_option", self._h, _capi.OPT_SWEEP, int(sweep))
+
     def set_mfma(self, on: bool) -> None:
         """Matrix-core kernels where the block shape has them (complex 8x8
         blocks; on by default), or the vector-unit kernels only."""
@@ -289,10 +294,11 @@ class HipEngine:
     def kernel_config(self, kind: int) -> dict:
         v = [C.c_int() for _ in range(4)]
         call("fkpm_get_kernel_config", self._h, int(kind), *[C.byref(x) for x in v])
-        mf = C.c_int()
+        mf, sw = C.c_int(), C.c_int()
         call("fkpm_get_kernel_mfma", self._h, int(kind), C.byref(mf))
+        call("fkpm_get_kernel_sweep", self._h, int(kind), C.byref(sw))
         return {"lanes": v[0].value, "kq": v[1].value, "n_wg": v[2].value, "nt": v[3].value,
-                "mfma": bool(mf.value), "mfma_variant": mf.value}
+                "mfma": bool(mf.value), "mfma_variant": mf.value, "sweep": sw.value}
 
     def state(self) -> dict:
         v = [C.c_int() for _ in range(3)]

@@ -178,6 +178,41 @@ def test_energy_scale_matches_oracle(oracle):
     assert es.lo <= lam[0] and es.hi >= lam[-1]
 
 
+@pytest.mark.parametrize("shape,b,ob,mfma", [((4, 4, 4), 2, False, False),
+                                            ((5, 3), 1, False, False),
+                                            ((3, 3, 2), 8, True, True),
+                                            ((3, 3, 2), 8, True, False)])
+def test_one_front_sweep_matches_oracle(oracle, shape, b, ob, mfma):
+    """All workgroups sweeping one front over the rows (instead of per-XCD
+    regions): vector-unit and matrix-core kernels against the oracle."""
+    from fastkpm_amd import lattice, EnergyScale
+    H = lattice.lattice(shape, b=b, J=1.0 if b == 2 else 0.0, W=1.0 if b < 8 else 0.0,
+                        orbital_blocks=ob, seed=13)
+    groups = lattice.colouring(shape, tuple(min(2, L) for L in shape), b)
+    R = seeded_R(oracle, groups, 17)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    lo, hi = lam[0] - 0.1, lam[-1] + 0.1
+    eng = _engine()
+    eng.set_mfma(mfma)
+    eng.set_sweep(1)
+    eng.set_R(R)
+    eng.set_H(H, EnergyScale(lo, hi))
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    M = 40
+    assert rel_err(eng.moments(M), orc.moments(M)) < FP64_TOL
+    for kind in (0, 2):
+        assert eng.kernel_config(kind)["sweep"] == 1
+        assert eng.kernel_config(kind)["mfma"] is mfma
+    c = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
+    D = H.like()
+    eng.autodiff_matrix(c, D)
+    assert rel_err(D.val, orc.autodiff_matrix(c, D)) < FP64_TOL
+    eng.stoch_matrix(c, D)
+    assert rel_err(D.val, orc.stoch_matrix(c, D)) < FP64_TOL
+
+
 def _ring_no_diagonal(nb=6):
     """A ring of 2x2 hopping blocks with no on-site blocks (the Lanczos
     pattern then gains explicit zero diagonal blocks)."""
