@@ -442,12 +442,15 @@ void plan(bool single, const Request& r, Layout* L) {
   const int need = (r.s + ne - 1) / ne;
   L->qp = ((need + L->W - 1) / L->W) * L->W;
   L->sp = L->qp * ne;
-  // workgroups per slot: one per 4 block rows up to 2048 (fixed by the shape)
+  // workgroups per slot: one per 4 block rows up to 2048 (fixed by the
+  // shape); they sweep the slot's rows as one front (chunk -1): a slot's
+  // first workgroup sits at any offset of the launch, so per-XCD regions
+  // would not line up with the XCDs
   int nwg = std::min(std::max((r.nb + 3) / 4, 1), 2048);
   int chunk = (r.nb + nwg - 1) / nwg;
   nwg = (r.nb + chunk - 1) / chunk;
   L->nwg = nwg;
-  L->chunk = chunk;
+  L->chunk = -1;
   const size_t vec = (size_t)r.n * L->sp * 2 * es;
   const size_t dbl = (size_t)r.nnzb * r.b * r.b * 2 * sizeof(double);
   const size_t h = r.M / 2;
