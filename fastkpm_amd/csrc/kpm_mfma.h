@@ -404,101 +404,6 @@ __device__ __forceinline__ void mfma_step_rr_dev(
   if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
 }
 
-// ---------------------------------------------------------------------------
-// Row-resident forward step with the next tile's loads in flight while the
-// current tile's MFMAs and epilogue run (two register buffers, tiles taken in
-// pairs).  Same contract and preconditions as mfma_step_rr_dev.
-// ---------------------------------------------------------------------------
-template <typename S, int DM = MF_DM>
-__device__ __forceinline__ void mfma_step_pp_dev(
-    WgId wid, const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
-    const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
-    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
-  using C2 = typename Mfma<S>::c2;
-  using V4 = typename Mfma<S>::v4;
-  using IO = BufC2<S>;
-  constexpr int NS = Vec16<S>::N;
-  constexpr uint32_t ES = sizeof(S);
-  (void)nt;
-  const MfmaLane m = mfma_lane<S>();
-  const int g = threadIdx.x / WAVE;
-  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
-  const uint32_t rs = (uint32_t)qp * NS;
-  const int ntile = (int)(rs / 32);
-  const uint32_t vbytes = 8u * (uint32_t)nb * rs * ES;
-  const __amdgpu_buffer_rsrc_t bx = vec_rsrc(x, vbytes), by = vec_rsrc(y, vbytes);
-  const __amdgpu_buffer_rsrc_t bz = vec_rsrc(z != nullptr ? z : x, z != nullptr ? vbytes : 0u);
-  const S acx = (S)cx, acz = (S)cz;
-  const uint32_t half = 4u * rs * ES;
-  double dmm = 0.0, dm1 = 0.0;
-
-  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
-    const int i = __builtin_amdgcn_readfirstlane(base + g);
-    if (i >= rr.r1) continue;
-    const int p0 = rp[i], deg = rp[i + 1] - p0;
-    S a[DM][4];
-    uint32_t o0[DM];
-#pragma unroll
-    for (int u = 0; u < DM; u++) {
-      if (deg > 0) {
-        const int pu = p0 + min(u, deg - 1);
-        const int j = ci[pu];
-        mfma_a_of<S>(mfma_hraw<S>(hv + (size_t)pu * 128, m), m, u < deg ? S(1) : S(0), a[u]);
-        o0[u] = ((uint32_t)(8 * j + m.q) * rs + 2u * m.c) * ES;
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) a[u][k] = S(0);
-        o0[u] = 0u;
-      }
-    }
-    const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 2u * m.c) * ES;
-    auto load = [&](int t, C2 (&g0)[DM], C2 (&g1)[DM], C2 (&e)[4]) {
-      const uint32_t so = (uint32_t)t * 32u * ES;
-#pragma unroll
-      for (int u = 0; u < DM; u++) {
-        g0[u] = IO::ld(bx, o0[u], so);
-        g1[u] = IO::ld(bx, o0[u] + half, so);
-      }
-      e[0] = IO::ld(bx, f0, so);
-      e[1] = IO::ld(bx, f0 + half, so);
-      e[2] = IO::ld(bz, f0, so);  // 0 without z
-      e[3] = IO::ld(bz, f0 + half, so);
-    };
-    auto finish = [&](int t, const C2 (&g0)[DM], const C2 (&g1)[DM], const C2 (&e)[4]) {
-      const uint32_t so = (uint32_t)t * 32u * ES;
-      V4 acc = V4{S(0), S(0), S(0), S(0)};
-#pragma unroll
-      for (int u = 0; u < DM; u++) acc = mfma_block<S>(a[u], g0[u], g1[u], acc);
-      C2 y0, y1;
-      y0.x = fma(acx, acc.x, acz * e[2].x);
-      y0.y = fma(acx, acc.y, acz * e[2].y);
-      y1.x = fma(acx, acc.z, acz * e[3].x);
-      y1.y = fma(acx, acc.w, acz * e[3].y);
-      if (rsrc.cr != 0.0) {
-        const int col = t * 16 + m.c;
-        mfma_rsource<S>(rsrc, (size_t)(8 * i + m.q), col, (f0 + so) / ES, y0);
-        mfma_rsource<S>(rsrc, (size_t)(8 * i + 4 + m.q), col, (f0 + half + so) / ES, y1);
-      }
-      IO::st(y0, by, f0, so);
-      IO::st(y1, by, f0 + half, so);
-      const double a0 = e[0].x, a1 = e[0].y, a2 = e[1].x, a3 = e[1].y;
-      dmm = fma(a0, a0, fma(a1, a1, fma(a2, a2, fma(a3, a3, dmm))));
-      dm1 = fma((double)y0.x, a0,
-                fma((double)y0.y, a1, fma((double)y1.x, a2, fma((double)y1.y, a3, dm1))));
-    };
-    C2 ga0[DM], ga1[DM], ea[4], gb0[DM], gb1[DM], eb[4];
-    load(0, ga0, ga1, ea);
-    for (int t = 0; t < ntile; t += 2) {
-      if (t + 1 < ntile) load(t + 1, gb0, gb1, eb);
-      finish(t, ga0, ga1, ea);
-      if (t + 1 >= ntile) break;
-      if (t + 2 < ntile) load(t + 2, ga0, ga1, ea);
-      finish(t + 1, gb0, gb1, eb);
-    }
-  }
-  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
-}
-
 // D(a, b) += fD * (G[a][b] + G[8+a][8+b], G[8+a][b] - G[a][8+b]) of block p from
 // the accumulator G in the MFMA output layout of S (see the file comment).
 template <typename S>
