@@ -80,7 +80,9 @@ __device__ __forceinline__ S* vec_a(const SlotDev& d, int k) {
 }
 
 // Forward Chebyshev step of every slot of the class in its forward phase.
-template <typename S, int B, int L, int KQ, bool MF>
+// MF: 0 vector units, 1 matrix cores, 2 matrix cores row-resident (rows of
+// at most MF_DM blocks, vectors below 4 GiB).
+template <typename S, int B, int L, int KQ, int MF>
 __global__ __launch_bounds__(WG) void k_batch_fwd(const SlotDev* __restrict__ sd,
                                                   const int* __restrict__ st,
                                                   const int* __restrict__ slots,
@@ -98,7 +100,10 @@ __global__ __launch_bounds__(WG) void k_batch_fwd(const SlotDev* __restrict__ sd
   const double cx = step == 0 ? 1.0 : 2.0, cz = step == 0 ? 0.0 : -1.0;
   double* part = d.partial + (size_t)step * d.nwg * 2;
   RSource<S> nors{nullptr, nullptr, nullptr, 0.0};
-  if constexpr (MF)
+  if constexpr (MF == 2)
+    mfma_step_rr_dev<S, MF_DM, 4>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp, d.chunk,
+                                  cx, cz, part, 0, nors);
+  else if constexpr (MF == 1)
     mfma_step_dev<S>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp, d.chunk, cx, cz, part,
                      0, nors);
   else
@@ -107,8 +112,9 @@ __global__ __launch_bounds__(WG) void k_batch_fwd(const SlotDev* __restrict__ sd
 }
 
 // Backward (autodiff) step of every slot of the class in its backward phase;
-// the sequence of autodiff_t in the single-request engine.
-template <typename S, int B, int L, int KQ, bool MF>
+// the sequence of autodiff_t in the single-request engine.  The row-resident
+// autodiff step is used in FP64 only (in FP32 the general one is faster).
+template <typename S, int B, int L, int KQ, int MF>
 __global__ __launch_bounds__(WG) void k_batch_bwd(const SlotDev* __restrict__ sd,
                                                   const int* __restrict__ st,
                                                   const int* __restrict__ slots,
@@ -131,7 +137,11 @@ __global__ __launch_bounds__(WG) void k_batch_bwd(const SlotDev* __restrict__ sd
   S* bout = (S*)d.Bv[m % 3];
   const double c4 = 4 * d.c[2 * m], c2 = 2 * d.c[2 * m + 1];
   const double w = m > 1 ? 2 * d.c[2 * m - 1] : 0.0;
-  if constexpr (MF)
+  if constexpr (MF == 2 && sizeof(S) == 8)
+    mfma_autodiff_rr_dev<S, true, MF_DM, 1>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2,
+                                            aout, bout, d.dacc, d.nb, d.qp, d.chunk, c4, c2, w,
+                                            d.sb, sb2, 2.0, 0);
+  else if constexpr (MF >= 1)
     mfma_autodiff_dev<S, true>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2, aout, bout,
                                d.dacc, d.nb, d.qp, d.chunk, c4, c2, w, d.sb, sb2, 2.0, 0);
   else
@@ -143,7 +153,7 @@ __global__ __launch_bounds__(WG) void k_batch_bwd(const SlotDev* __restrict__ sd
 // Auxiliary work of a tick: zero D (tick 0), D = diag R R^+ (tick M/2),
 // D += alpha_0 beta_1^+ (tick M), symmetrize/scale D and reduce the moment
 // dots (tick M+1).
-template <typename S, int B, int L, int KQ, bool MF>
+template <typename S, int B, int L, int KQ, int MF>
 __global__ __launch_bounds__(WG) void k_batch_aux(const SlotDev* __restrict__ sd,
                                                   const int* __restrict__ st,
                                                   const int* __restrict__ slots,
@@ -315,22 +325,25 @@ struct ClassDef {
   BatchKernel fwd, bwd, aux;
 };
 
-template <typename S, int B, int L, int KQ, bool MF>
+template <typename S, int B, int L, int KQ, int MF>
 ClassDef mk_class(int W) {
-  return ClassDef{B, L, KQ, MF ? 1 : 0, W, &fkd::k_batch_fwd<S, B, L, KQ, MF>,
+  return ClassDef{B, L, KQ, MF, W, &fkd::k_batch_fwd<S, B, L, KQ, MF>,
                   &fkd::k_batch_bwd<S, B, L, KQ, MF>, &fkd::k_batch_aux<S, B, L, KQ, MF>};
 }
 
 // Complex 2x2 blocks: pass widths 16, 32, 64 packets; complex 8x8 blocks:
 // matrix cores (tiles of 16 columns) with the outer products at width 16
-// (FP64) or 8 (FP32) packets.
+// (FP64) or 8 (FP32) packets, general and row-resident (the last entry, taken
+// for requests whose rows all have at most MF_DM blocks).
 const std::vector<ClassDef>& classes(bool single) {
   static const std::vector<ClassDef> d = {
-      mk_class<double, 2, 8, 2, false>(16), mk_class<double, 2, 16, 2, false>(32),
-      mk_class<double, 2, 64, 1, false>(64), mk_class<double, 8, 16, 1, true>(16)};
+      mk_class<double, 2, 8, 2, 0>(16), mk_class<double, 2, 16, 2, 0>(32),
+      mk_class<double, 2, 64, 1, 0>(64), mk_class<double, 8, 16, 1, 1>(16),
+      mk_class<double, 8, 16, 1, 2>(16)};
   static const std::vector<ClassDef> f = {
-      mk_class<float, 2, 8, 2, false>(16), mk_class<float, 2, 16, 2, false>(32),
-      mk_class<float, 2, 64, 1, false>(64), mk_class<float, 8, 8, 1, true>(8)};
+      mk_class<float, 2, 8, 2, 0>(16), mk_class<float, 2, 16, 2, 0>(32),
+      mk_class<float, 2, 64, 1, 0>(64), mk_class<float, 8, 8, 1, 1>(8),
+      mk_class<float, 8, 8, 1, 2>(8)};
   return single ? f : d;
 }
 
@@ -386,6 +399,7 @@ struct Request {
   int nb = 0, b = 0, nnzb = 0, n = 0, s = 0, M = 0;
   double lo = 0, hi = 0;
   std::vector<int> rp, ci, rowblk, tpos;
+  int max_deg = 0;  // most blocks in a row
   std::vector<double> val, R, c;
   bool compact = false;  // R given as (rcol, rval) per orbital row (R holds rval)
   bool seeded = false;   // compact R generated on the device: rcol = colours
@@ -411,13 +425,13 @@ struct Layout {
 enum Buf { B_RP, B_CI, B_RB, B_TP, B_HV, B_HRAW, B_R, B_A0, B_A1, B_A2, B_B0, B_B1, B_B2, B_RRAW,
            B_DACC, B_DOUT, B_PART, B_PSUM, B_C, B_COUNT };
 
-int pick_class(bool single, int b, int s, int* W) {
+int pick_class(bool single, int b, int s, bool row_resident, int* W) {
   const auto& cl = classes(single);
   const int ne = single ? 2 : 1;  // complex elements per 16-byte packet
   const int need = (s + ne - 1) / ne;
   int best = -1;
   for (int k = 0; k < (int)cl.size(); k++) {
-    if (cl[k].B != b) continue;
+    if (cl[k].B != b || (cl[k].mf == 2) != (b == 8 && row_resident)) continue;
     if (best < 0) {
       best = k;
       continue;
@@ -433,13 +447,19 @@ int pick_class(bool single, int b, int s, int* W) {
 
 void plan(bool single, const Request& r, Layout* L) {
   const size_t es = single ? 4 : 8;
-  L->cls = pick_class(single, r.b, r.s, &L->W);
+  const int ne = single ? 2 : 1;
+  const int need = (r.s + ne - 1) / ne;
+  // row-resident matrix-core kernels: rows of at most MF_DM blocks and
+  // vector byte offsets within 32 bits (at the class's pass width of 16 / 8
+  // packets)
+  const int wb8 = single ? 8 : 16;
+  const size_t vb8 = (size_t)r.n * ((need + wb8 - 1) / wb8) * wb8 * ne * 2 * es;
+  const bool rr = r.b == 8 && r.max_deg <= fkd::MF_DM && vb8 < ((size_t)1 << 32);
+  L->cls = pick_class(single, r.b, r.s, rr, &L->W);
   if (L->cls < 0) {
     L->total = 0;
     return;
   }
-  const int ne = single ? 2 : 1;
-  const int need = (r.s + ne - 1) / ne;
   L->qp = ((need + L->W - 1) / L->W) * L->W;
   L->sp = L->qp * ne;
   // workgroups per slot: one per 4 block rows up to 2048 (fixed by the
@@ -1020,6 +1040,7 @@ static int submit_common(fkpm_batch* B, int nb, int b, int nnzb, const int* row_
       r->rowblk[p] = i;
     }
     if (!diag) return berr(FKPM_E_ARG, "batch_submit: diagonal block %d missing", i);
+    r->max_deg = std::max(r->max_deg, r->rp[i + 1] - r->rp[i]);
   }
   // transpose positions (structurally symmetric pattern, as symmetrize needs)
   r->tpos.assign(nnzb, -1);

@@ -162,6 +162,52 @@ def test_batch_seeded_equals_host_probing(shape, b, per):
     eng.close()
 
 
+def _dense_rows_b8(nb=20, extra=6, seed=3):
+    """Hermitian 8x8-block matrix with rows of up to 2*extra+1 blocks: more
+    than the row-resident matrix-core kernels take."""
+    from fastkpm_amd import lattice
+    rng = np.random.default_rng(seed)
+    rows, cols, blocks = [], [], []
+    for i in range(nb):
+        x = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+        rows.append(i)
+        cols.append(i)
+        blocks.append((x + x.conj().T) / 2)
+        for j in rng.choice([k for k in range(nb) if k != i], size=extra, replace=False):
+            t = 0.5 * (rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8)))
+            rows += [i, int(j)]
+            cols += [int(j), i]
+            blocks += [t, t.conj().T]
+    return lattice.coo_to_bsr(nb, 8, rows, cols, np.array(blocks))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_b8_row_resident_and_general(oracle, prec):
+    """8x8-block requests take the row-resident matrix-core class when every
+    row has at most 8 blocks and the general one otherwise; both in flight
+    together, each against the oracle."""
+    from fastkpm_amd import EnergyScale
+    from fastkpm_amd.serving import BatchEngine
+    Hd = _dense_rows_b8()
+    assert int(np.max(np.diff(Hd.row_ptr))) > 8
+    lam = np.linalg.eigvalsh(Hd.to_dense())
+    esd = EnergyScale(lam[0] - 0.2, lam[-1] + 0.2)
+    groups = (np.arange(8)[None, :] + 8 * (np.arange(Hd.n_rows) % 3)[:, None]).reshape(-1)
+    Rd = seeded_R(oracle, groups.astype(np.int32), 41)
+    Md = 28
+    cd = fermi_coeffs(oracle, Md, esd.lo, esd.hi, kT=0.05)
+    Hl, esl, cl, Rl, _ = _req(oracle, "b8", 43, 36)
+    eng = BatchEngine(prec, hbm_bytes=1 << 30, max_slots=4)
+    ids = [eng.submit(Hd, esd, cd, R=Rd), eng.submit(Hl, esl, cl, R=Rl)]
+    tol_mu, tol_d = (FP64_TOL, FP64_TOL) if prec == "fp64" else (2e-5, 1e-4)
+    for (H, es, c, R), rid in zip([(Hd, esd, cd, Rd), (Hl, esl, cl, Rl)], ids):
+        r = eng.wait(rid)
+        mu_o, D_o = _oracle_solve(oracle, H, es, c, R)
+        assert rel_err(r.mu, mu_o) < tol_mu
+        assert rel_err(r.D, D_o) < tol_d
+    eng.close()
+
+
 def test_batch_refusals(oracle):
     from fastkpm_amd import lattice, EnergyScale
     from fastkpm_amd._capi import FkpmError
