@@ -152,7 +152,8 @@ struct KCfg {
   int L = 0, KQ = 0, maxwg = 0, nt = 0;
   int mfma = 0;  // matrix-core kernel (one wave per block row: L = 64, KQ = 1):
                  // 1 general, 2 row-resident (rows of <= 8 blocks, vectors < 4 GiB)
-  int sweep = 0;  // rows: 0 per-XCD regions, 1 one front over all rows (chunk = -1)
+  int sweep = 0;  // rows: 0 per-XCD regions, s >= 1 one front in segments of s passes
+                  // per XCD (chunk = -s, workgroups a multiple of 8 s)
   int nwg = 0, chunk = 0;
 };
 
@@ -275,7 +276,13 @@ void derive_grid(int nb, KCfg& c) {
   int chunk = (nb + nwg - 1) / nwg;
   chunk = ((chunk + gwg - 1) / gwg) * gwg;
   c.nwg = (nb + chunk - 1) / chunk;
-  c.chunk = c.sweep ? -1 : chunk;
+  c.chunk = chunk;
+  if (c.sweep > 0) {
+    // one front: a multiple of 8 s workgroups (the extra ones find no rows)
+    const int q = 8 * c.sweep;
+    c.nwg = (c.nwg + q - 1) / q * q;
+    c.chunk = -c.sweep;
+  }
 }
 
 // Matrix-core kernels exist for complex 8x8 blocks (forward/Clenshaw steps
@@ -559,6 +566,7 @@ int tune_t(fkpm_handle* h) {
   const int ng = fkd::geometries_for_block(h->b, gs, 64);
   static const int caps[] = {1024, 4096, 16384};
   static const int nts[] = {0, fkd::NT_LOAD | fkd::NT_STORE};
+  static const int sweeps[] = {0, 1, 2, 8};
   const int kinds[] = {K_MOM, K_XI, K_AD};
   for (int kind : kinds) {
     KCfg best = h->kc[kind];
@@ -569,7 +577,7 @@ int tune_t(fkpm_handle* h) {
       if (w > h->wmax || 2 * w < h->wmax) continue;
       for (int cap : caps)
         for (int nt : nts)
-          for (int sw = 0; sw < 2; sw++) {
+          for (int sw : sweeps) {
             KCfg c;
             c.L = gs[g].L;
             c.KQ = gs[g].KQ;
@@ -582,7 +590,7 @@ int tune_t(fkpm_handle* h) {
     if (mfma_avail(h, kind))
       for (int v = 1; v <= (mfma_rr_avail(h, kind) ? 2 : 1); v++)
         for (int cap : caps)
-          for (int sw = 0; sw < 2; sw++) {
+          for (int sw : sweeps) {
             KCfg c;
             c.mfma = v;
             c.maxwg = cap;
@@ -1690,7 +1698,7 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
       h->autotune_on_use = value != 0;
       return 0;
     case FKPM_OPT_SWEEP:
-      if (value < -1 || value > 1) return fail(FKPM_E_ARG, "bad sweep %d", value);
+      if (value < -1 || value > 64) return fail(FKPM_E_ARG, "bad sweep %d", value);
       h->sweep_force = value;
       if (value >= 0)
         for (int k = 0; k < NKIND; k++)

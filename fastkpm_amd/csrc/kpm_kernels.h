@@ -287,17 +287,22 @@ struct WgId {
 __device__ __forceinline__ WgId wg_self() { return WgId{(int)blockIdx.x, (int)gridDim.x}; }
 
 //
-// chunk < 0 selects the other sweep: all workgroups advance one front over
-// all rows (workgroup k takes passes k, k + n, ...).  A neighbour d rows away
-// is then d / step workgroups later, on the same XCD when that is a multiple
-// of 8, and the rows in flight across the chip span a narrow window whose
-// far neighbours (a 3-D lattice's next plane) stay in the shared last-level
-// cache rather than eight separate windows.
+// chunk = -s (s >= 1) selects the other sweep: all workgroups advance one
+// front over all rows, the rows cut into segments of s passes (s * step
+// rows) dealt to the XCDs in turn (segment j to XCD j % 8); the workgroups of
+// an XCD take its passes round-robin.  With n a multiple of 8 s, workgroup
+// (x = k % 8, r = k / 8) starts at ((r / s) * 8 + x) * s * step + (r % s) * step
+// and advances by n * step.  The rows in flight across the chip span one
+// narrow window, whose far neighbours (a 3-D lattice's next plane) stay in the
+// shared last-level cache, a neighbour d rows away is on the same XCD when
+// d / (s * step) is a multiple of 8, and only rows at segment edges have a
+// near neighbour on another XCD (s = 1: workgroup k takes passes k, k + n, ...).
 __device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step, WgId wid) {
   const int k = wid.k, n = wid.n;
   if (chunk < 0) {
+    const int sp = -chunk, x = k & 7, r = k >> 3;
     RowRange rr;
-    rr.r0 = k * step;
+    rr.r0 = ((r / sp) * 8 + x) * sp * step + (r % sp) * step;
     rr.r1 = nb;
     rr.stride = n * step;
     return rr;

@@ -157,9 +157,10 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 #define FKPM_OPT_AUTOTUNE 2     /* value 1: tune the kernel geometry at first use of a shape */
 #define FKPM_OPT_MFMA 3         /* value 1 (default): matrix-core (MFMA) kernels where the block
                                    shape has them (complex 8x8 blocks); 0: vector-unit kernels */
-#define FKPM_OPT_SWEEP 4        /* value 0 / 1: every kernel kind sweeps the rows per XCD / as
-                                   one front (fkpm_get_kernel_sweep); -1 (default): the
-                                   configuration's own (the tuner's choice, else per XCD) */
+#define FKPM_OPT_SWEEP 4        /* value 0: every kernel kind sweeps the rows per XCD; s in
+                                   1..64: as one front in segments of s passes per XCD
+                                   (fkpm_get_kernel_sweep); -1 (default): the configuration's
+                                   own (the tuner's choice, else per XCD) */
 #define FKPM_STOCH_CLENSHAW 0   /* xi by the Clenshaw recurrence: one 2-vector pass per order */
 #define FKPM_STOCH_FORWARD 1    /* xi accumulated in FP64 along the forward recursion, the
                                    reference's summation order (engine_cpu.cpp:236-244) */
@@ -188,8 +189,9 @@ int fkpm_get_kernel_mfma(fkpm_handle* h, int kind, int* mfma);
 
 /* How the workgroups of a kernel kind cover the block rows: 0 = each of the 8
  * XCDs sweeps its own contiguous region (neighbour rows stay in that XCD's
- * L2), 1 = all workgroups sweep one front over all rows (far neighbours stay
- * in the shared last-level cache).  The autotuner times both. */
+ * L2); s >= 1 = all workgroups sweep one front over all rows (far neighbours
+ * stay in the shared last-level cache), the rows dealt to the XCDs in
+ * segments of s workgroup passes.  The autotuner times 0, 1, 2 and 8. */
 int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep);
 
 /* r_compact: R has at most one nonzero per row (probing colours) and the

@@ -178,13 +178,16 @@ def test_energy_scale_matches_oracle(oracle):
     assert es.lo <= lam[0] and es.hi >= lam[-1]
 
 
-@pytest.mark.parametrize("shape,b,ob,mfma", [((4, 4, 4), 2, False, False),
-                                            ((5, 3), 1, False, False),
-                                            ((3, 3, 2), 8, True, True),
-                                            ((3, 3, 2), 8, True, False)])
-def test_one_front_sweep_matches_oracle(oracle, shape, b, ob, mfma):
+@pytest.mark.parametrize("shape,b,ob,mfma,sweep", [((4, 4, 4), 2, False, False, 1),
+                                                  ((6, 6, 4), 2, False, False, 8),
+                                                  ((5, 3), 1, False, False, 2),
+                                                  ((3, 3, 2), 8, True, True, 1),
+                                                  ((3, 3, 4), 8, True, True, 8),
+                                                  ((3, 3, 2), 8, True, False, 2)])
+def test_one_front_sweep_matches_oracle(oracle, shape, b, ob, mfma, sweep):
     """All workgroups sweeping one front over the rows (instead of per-XCD
-    regions): vector-unit and matrix-core kernels against the oracle."""
+    regions), in segments of 1, 2 or 8 passes per XCD: vector-unit and
+    matrix-core kernels against the oracle."""
     from fastkpm_amd import lattice, EnergyScale
     H = lattice.lattice(shape, b=b, J=1.0 if b == 2 else 0.0, W=1.0 if b < 8 else 0.0,
                         orbital_blocks=ob, seed=13)
@@ -194,7 +197,7 @@ def test_one_front_sweep_matches_oracle(oracle, shape, b, ob, mfma):
     lo, hi = lam[0] - 0.1, lam[-1] + 0.1
     eng = _engine()
     eng.set_mfma(mfma)
-    eng.set_sweep(1)
+    eng.set_sweep(sweep)
     eng.set_R(R)
     eng.set_H(H, EnergyScale(lo, hi))
     orc = oracle.Engine()
@@ -203,7 +206,7 @@ def test_one_front_sweep_matches_oracle(oracle, shape, b, ob, mfma):
     M = 40
     assert rel_err(eng.moments(M), orc.moments(M)) < FP64_TOL
     for kind in (0, 2):
-        assert eng.kernel_config(kind)["sweep"] == 1
+        assert eng.kernel_config(kind)["sweep"] == sweep
         assert eng.kernel_config(kind)["mfma"] is mfma
     c = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
     D = H.like()
