@@ -1057,16 +1057,24 @@ int moments2_t(fkpm_handle* h, int M, int col_chunk, size_t budget, double* mu) 
   if ((size_t)(2 * M + 3) * vec_bytes(spc) > budget)
     return fail(FKPM_E_ARG, "moments2: M = %d vectors of %zu bytes exceed the %zu-byte budget", M,
                 vec_bytes(spc), budget);
+  // the engine's own forward-step configuration (tuned or default) when it
+  // has this pass width, else the widest lane groups of that width
   fkd::KSet<S> ks;
-  int L = 0, KQ = 0;
-  if (!kset_for_width<S>(h, W, &ks, &L, &KQ))
-    return fail(FKPM_E_UNSUP, "moments2: no kernel of width %d for block %d", W, h->b);
-  const int qpc = spc / ne;
   KCfg c;
-  c.L = L;
-  c.KQ = KQ;
-  c.maxwg = 4096;
-  derive_grid(h->nb, c);
+  const KCfg& km = h->kc[K_MOM];
+  if (!km.mfma && km.L * km.KQ == W) {
+    ks = h->ks<S>(K_MOM);
+    c = km;
+  } else {
+    int L = 0, KQ = 0;
+    if (!kset_for_width<S>(h, W, &ks, &L, &KQ))
+      return fail(FKPM_E_UNSUP, "moments2: no kernel of width %d for block %d", W, h->b);
+    c.L = L;
+    c.KQ = KQ;
+    c.maxwg = 4096;
+    derive_grid(h->nb, c);
+  }
+  const int qpc = spc / ne;
   const size_t vb = vec_bytes(spc);
   // the columns of A and B are spaced by a vector plus 4352 bytes: with K a
   // power of two a plain spacing would put the same k of every column in the
@@ -1088,7 +1096,7 @@ int moments2_t(fkpm_handle* h, int M, int col_chunk, size_t budget, double* mu) 
   auto step = [&](const int* rp_, const int* ci_, const S* hv_, const S* x, const S* z, S* y,
                   double cx, double cz) {
     return launch(ks.mom, grid, block, h->stream, rp_, ci_, hv_, x, z, y, h->nb, qpc, c.chunk,
-                  cx, cz, (double*)nullptr, 0, no_rsource<S>());
+                  cx, cz, (double*)nullptr, c.nt, no_rsource<S>());
   };
   const int* j1rp = h->j_rp[0].as<const int>();
   const int* j1ci = h->j_ci[0].as<const int>();
