@@ -216,6 +216,14 @@ def test_one_front_sweep_matches_oracle(oracle, shape, b, ob, mfma, sweep):
     assert rel_err(D.val, orc.stoch_matrix(c, D)) < FP64_TOL
 
 
+def test_option_refusals():
+    from fastkpm_amd._capi import FkpmError, OPT_SWEEP, OPT_MFMA, call
+    eng = _engine()
+    for opt, val in ((OPT_SWEEP, 65), (OPT_SWEEP, -2), (OPT_MFMA, 2), (99, 0)):
+        with pytest.raises(FkpmError):
+            call("fkpm_set_option", eng.handle, opt, val)
+
+
 def _ring_no_diagonal(nb=6):
     """A ring of 2x2 hopping blocks with no on-site blocks (the Lanczos
     pattern then gains explicit zero diagonal blocks)."""
