@@ -96,8 +96,11 @@ def measured_traffic(config: str, prec: str, kernel: str):
     data = json.loads(path.read_text())
     for k in data.get("kernels", []):
         if kernel.split(" ")[0] in k["kernel"]:
-            return {"traffic_bytes_per_launch": k["traffic_bytes_per_launch"],
-                    "source": f"{path.name} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"}
+            out = {"traffic_bytes_per_launch": k["traffic_bytes_per_launch"],
+                   "source": f"{path.name} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"}
+            if "request_bytes_per_launch" in k:
+                out["request_bytes_per_launch"] = k["request_bytes_per_launch"]
+            return out
     return None
 
 
@@ -478,6 +481,9 @@ def main(argv=None) -> int:
     if tr is not None:
         roof["traffic"] = tr["traffic_bytes_per_launch"]
         roof["traffic_source"] = tr["source"]
+        if "request_bytes_per_launch" in tr:
+            # the same bytes from the EA request counters by size (32/64/128 B)
+            roof["traffic_by_request_size"] = tr["request_bytes_per_launch"]
     times_ms = np.array(r["times"]) * 1e3
     out = {
         "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,
