@@ -95,7 +95,7 @@ def measured_traffic(config: str, prec: str, kernel: str):
         return None
     data = json.loads(path.read_text())
     for k in data.get("kernels", []):
-        if kernel.split(" ")[0] in k["kernel"]:
+        if "::" + kernel.split(" ")[0] + "<" in k["kernel"]:
             out = {"traffic_bytes_per_launch": k["traffic_bytes_per_launch"],
                    "source": f"{path.name} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"}
             if "request_bytes_per_launch" in k:
@@ -158,12 +158,13 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     eng.get_D(Ds)  # the device holds the last computed D: stoch_matrix's
     # copy of the gradient D: one more moments + autodiff_matrix to the host
     eng.moments(M)
+    alpha_table = eng.alpha_table()  # autodiff reads the kept forward vectors
     eng.autodiff_matrix(cE, Dg)
     kcfg = {name: eng.kernel_config(k) for k, name in
             ((0, "moments"), (1, "xi"), (2, "autodiff"), (3, "outer"))}
     return dict(eng=eng, es=es, N=N, S=S, M=M, mu=mu, energy=energy, wall=wall, times=times,
                 kernel_ms=tot, Dg=Dg.val.copy(), Ds=Ds.val.copy(), config=eng.config(),
-                kernel_config=kcfg, tune_s=t_tune, state=eng.state(),
+                kernel_config=kcfg, tune_s=t_tune, state=eng.state(), alpha_table=alpha_table,
                 energy_scale_s=None if "es" in cfg else t_es)
 
 
@@ -456,7 +457,7 @@ def main(argv=None) -> int:
         geo = f"b={H.b_len}, matrix cores, wg={kc['n_wg']}"
     else:
         names = {"moments": "k_step_moments", "stoch": "k_step_moments (Clenshaw)",
-                 "autodiff": "k_step_autodiff"}
+                 "autodiff": "k_step_autodiff_stored" if r["alpha_table"] else "k_step_autodiff"}
         geo = f"b={H.b_len}, lanes={kc['lanes']}, kq={kc['kq']}, wg={kc['n_wg']}, nt={kc['nt']}"
     roof = {"bound": "hbm", "achieved": kern[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None,

@@ -201,6 +201,9 @@ struct fkpm_handle {
   int stoch_method = FKPM_STOCH_CLENSHAW;
   DevBuf R, A[3], Bv[3], xi, partial, psum, tune_dacc;
   int alpha_M = 0;  // M of the moments whose final state is held, 0 if none
+  DevBuf atab;            // alpha_1 .. alpha_{M/2} of the last moments(M), when it fits
+  bool a_stored = false;  // the forward vectors live in atab (else the ring A[3])
+  int store_alpha = 1;    // FKPM_OPT_STORE_ALPHA: 0 never, 1 auto, 2 whenever the table fits
   int sweep_force = -1;  // FKPM_OPT_SWEEP: row sweep of every kind (-1: per configuration)
 
   // --- D ---
@@ -224,7 +227,7 @@ struct fkpm_handle {
     DevBuf* bufs[] = {&rp,    &ci,    &hv,      &rowblk, &hraw, &rraw,  &rcol,     &rval,
                       &rflag, &R,     &A[0],    &A[1],   &A[2], &Bv[0], &Bv[1],    &Bv[2],
                       &xi,    &partial, &psum,  &tune_dacc, &drp, &dci,  &tpos,     &dacc,
-                      &dout};
+                      &dout,  &atab};
     for (DevBuf* d : bufs) d->release();
     DevBuf* m2bufs[] = {&j_rp[0], &j_ci[0], &j_rb[0], &j_raw[0], &j_val[0], &j_rp[1], &j_ci[1],
                         &j_rb[1], &j_raw[1], &j_val[1], &m2_a, &m2_b, &m2_ring, &m2_c, &m2_part};
@@ -237,7 +240,13 @@ struct fkpm_handle {
   size_t vbytes() const { return (size_t)n * sp * (cx ? 2 : 1) * (single ? 4 : 8); }
   size_t accbytes() const { return (size_t)n * sp * (cx ? 2 : 1) * 8; }
   int ne() const { return (single ? 4 : 2) / (cx ? 2 : 1); }  // elements per 16-byte packet
-  void* vec_a(int k) { return k == 0 ? R.p : A[k % 3].p; }  // forward-phase alpha_k
+  // forward-phase alpha_k: R, then the table of every alpha_k (a_stored) or
+  // the ring of three
+  void* vec_a(int k) {
+    if (k == 0) return R.p;
+    if (a_stored) return static_cast<char*>(atab.p) + (size_t)(k - 1) * vbytes();
+    return A[k % 3].p;
+  }
   template <class S>
   const fkd::KSet<S>& ks(int kind) const;
 };
@@ -667,6 +676,27 @@ template <class S>
 int moments_t(fkpm_handle* h, int M, double* mu) {
   const int nsteps = M / 2;
   CHK(ensure_partials(h, nsteps));
+  // keep every alpha_k for autodiff_matrix (no backward recomputation of
+  // alpha) when the table fits half the free device memory (or the byte
+  // count in FKPM_ALPHA_TABLE_BYTES) and the autodiff step runs on the
+  // vector units.  Automatically only when the three-vector ring would not
+  // fit the 256 MB last-level cache anyway: below that the forward steps
+  // run faster on the ring (measured: FP32 C2 forward step 37 -> 49 us on a
+  // table, autodiff 120 -> 110 us).
+  {
+    const size_t need = (size_t)nsteps * h->vbytes();
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    size_t budget = free_b / 2 + h->atab.bytes;
+    if (const char* e = getenv("FKPM_ALPHA_TABLE_BYTES")) budget = (size_t)strtoull(e, nullptr, 10);
+    const bool want = h->store_alpha == 2 ||
+                      (h->store_alpha == 1 && 3 * h->vbytes() > ((size_t)256 << 20));
+    h->a_stored = false;
+    if (want && !h->kc[K_AD].mfma && need <= budget) {
+      CHK(h->atab.ensure(need));
+      h->a_stored = true;
+    }
+  }
   const KCfg& kc = h->kc[K_MOM];
   const auto& ks = h->ks<S>(K_MOM);
   const dim3 grid(kc.nwg), block(fkd::WG);
@@ -770,6 +800,7 @@ int stoch_clenshaw(fkpm_handle* h, int M, const double* c, const S** xi_out) {
 
 template <class S>
 int stoch_t(fkpm_handle* h, int M, const double* c) {
+  h->a_stored = false;  // the forward method's alpha_k go to the ring
   const dim3 block(fkd::WG);
   const int* rp = h->rp.as<const int>();
   const int* ci = h->ci.as<const int>();
@@ -852,10 +883,13 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
     const size_t vb = h->vbytes();
     for (int k = 0; k < 3; k++) CHK(h->Bv[k].ensure(vb));
     const double sb = 2 * c[M - 1];  // beta_{M/2} = 2 c_{M-1} alpha_{M/2-1}
+    const bool stored = h->a_stored && !ka.mfma;
     for (int m = M / 2 - 1; m >= 1; m--) {
-      const S* am = (const S*)h->A[m % 3].p;       // alpha_m
-      const S* am1 = (const S*)h->A[(m + 1) % 3].p;  // alpha_{m+1}
-      S* aout = (S*)h->A[(m - 1) % 3].p;             // alpha_{m-1}
+      // alpha_m, alpha_{m+1} and alpha_{m-1}: from the table, or the ring with
+      // alpha_{m-1} recomputed by the step
+      const S* am = (const S*)(stored ? h->vec_a(m) : h->A[m % 3].p);
+      const S* am1 = (const S*)(stored ? h->vec_a(m + 1) : h->A[(m + 1) % 3].p);
+      S* aout = stored ? (S*)h->vec_a(m - 1) : (S*)h->A[(m - 1) % 3].p;
       const bool first = (m == M / 2 - 1);
       const bool second = (m == M / 2 - 2);
       const S* bm1 = first ? nullptr : (const S*)h->Bv[(m + 1) % 3].p;
@@ -864,7 +898,14 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
       S* bout = (S*)h->Bv[m % 3].p;
       const double c4 = 4 * c[2 * m], c2 = 2 * c[2 * m + 1];
       const double w = m > 1 ? 2 * c[2 * m - 1] : 0.0;
-      if (h->d_fused) {
+      if (stored) {
+        if (!h->d_fused)  // D += 2 alpha_m beta_{m+1}^+ on D's own pattern
+          CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, am, first ? am : bm1,
+                     dacc, h->nb, h->qp, ko.chunk, first ? 2.0 * sb : 2.0));
+        CHK(launch(h->d_fused ? ksa.ad_stored : ksa.ad_stored_plain, dim3(ka.nwg), block,
+                   h->stream, rp, ci, hv, am, am1, bm1, bm2, (const S*)aout, bout, dacc, h->nb,
+                   h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
+      } else if (h->d_fused) {
         CHK(launch(adf_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
                    aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
       } else {
@@ -876,9 +917,10 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
       }
       launches++;
     }
-    // D += alpha_0 beta_1^+
-    CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, (const S*)h->A[0].p,
-               (const S*)h->Bv[1].p, dacc, h->nb, h->qp, ko.chunk, 1.0));
+    // D += alpha_0 beta_1^+ (alpha_0 = R kept, or recomputed into A[0])
+    CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci,
+               (const S*)(stored ? h->vec_a(0) : h->A[0].p), (const S*)h->Bv[1].p, dacc, h->nb,
+               h->qp, ko.chunk, 1.0));
   }
   CHK(timing_end(h, &h->ms_ad, &h->n_ad, launches));
   // symmetrize + scale(1/mag)
@@ -1656,6 +1698,12 @@ int fkpm_get_kernel_mfma(fkpm_handle* h, int kind, int* mfma) {
   return 0;
 }
 
+int fkpm_get_alpha_table(fkpm_handle* h, int* stored) {
+  if (!h || !stored) return fail(FKPM_E_ARG, "null argument");
+  *stored = (h->a_stored && h->alpha_M > 0) ? 1 : 0;
+  return 0;
+}
+
 int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep) {
   if (!h || !sweep) return fail(FKPM_E_ARG, "null argument");
   if (kind < 0 || kind >= NKIND) return fail(FKPM_E_ARG, "bad kernel kind %d", kind);
@@ -1704,6 +1752,12 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
       return 0;
     case FKPM_OPT_AUTOTUNE:
       h->autotune_on_use = value != 0;
+      return 0;
+    case FKPM_OPT_STORE_ALPHA:
+      if (value < 0 || value > 2) return fail(FKPM_E_ARG, "bad store-alpha mode %d", value);
+      h->store_alpha = value;
+      h->alpha_M = 0;  // moments(M) again before autodiff_matrix
+      h->a_stored = false;
       return 0;
     case FKPM_OPT_SWEEP:
       if (value < -1 || value > 64) return fail(FKPM_E_ARG, "bad sweep %d", value);

@@ -24,6 +24,14 @@ struct KSet {
   void (*ad_plain)(const int*, const int*, const S*, const S*, const S*, const S*, const S*, S*,
                    S*, double*, int, int, int, double, double, double, double, double,
                    double, int) = nullptr;
+  // backward steps reading alpha_{m-1} from the forward table (D fused on H's
+  // pattern, or not)
+  void (*ad_stored)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
+                    const S*, S*, double*, int, int, int, double, double, double, double, double,
+                    double, int) = nullptr;
+  void (*ad_stored_plain)(const int*, const int*, const S*, const S*, const S*, const S*,
+                          const S*, const S*, S*, double*, int, int, int, double, double, double,
+                          double, double, double, int) = nullptr;
   void (*outer_ss)(const int*, const int*, const S*, const S*, double*, int, int, int,
                    double) = nullptr;
   void (*outer_sd)(const int*, const int*, const S*, const double*, double*, int, int, int,

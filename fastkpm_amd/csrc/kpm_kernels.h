@@ -776,6 +776,175 @@ __device__ __forceinline__ void step_autodiff_dev(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Backward step with the forward vectors kept: when moments(M) left every
+// alpha_k in HBM (a table of M/2 vectors), alpha_{m-1} is read instead of
+// recomputed, so the step gathers beta_{m+1} only:
+//   b_out = beta_m = 4 c_{2m} alpha_m + 2 c_{2m+1} alpha_{m+1}
+//                    + 2 Hs beta_{m+1} - beta_{m+2} + w alpha_{m-1}
+//   D    += 2 alpha_m beta_{m+1}^+        (FUSED_D: on H's own pattern)
+// with am, am1, amm = alpha_m, alpha_{m+1}, alpha_{m-1} (own rows only).
+// The same algorithmic bytes as k_step_autodiff (five vectors read, one
+// written) but one gathered vector instead of two, so half the neighbour
+// rows compete for the L2.  bm1 == null: beta_{m+1} = sb * alpha_m (its
+// gather reads alpha_m); bm2 == null: beta_{m+2} = sb2 * alpha_{m+1}.
+// ---------------------------------------------------------------------------
+template <typename S, bool CX, int B, int L, int KQ, bool FUSED_D>
+__device__ __forceinline__ void step_autodiff_stored_dev(
+    WgId wid,
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, const S* __restrict__ amm, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  using A = S;
+  using GE = Geo<S, CX, B, L, KQ>;
+  constexpr int NS = GE::NS;
+  constexpr int HS = GE::HS;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = lane % L;
+  const int g = (threadIdx.x / WAVE) * GE::G + lane / L;
+  const RowRange rr = wg_rows(nb, chunk, GE::GWG, wid);
+  const size_t rs = (size_t)qp * NS;
+  const bool vb1 = (bm1 == nullptr);
+  const double hb1 = vb1 ? sb : 1.0;  // scale of the beta_{m+1} gather
+  const S* const X[1] = {vb1 ? am : bm1};
+  const A ab2 = (A)(2.0 * hb1), ac4 = (A)c4, ac2 = (A)c2, aw = (A)w, asb2 = (A)sb2;
+  const double fD = dscale * hb1;
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    int i = base + g;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+      const size_t qo = (size_t)(q0 + l) * NS;
+      A ai[KQ][B][NS];
+      A acc[1][KQ][B][NS];
+      load_rows<S, A, B, L, KQ>(am, i, rs, qo, ai);
+#pragma unroll
+      for (int k = 0; k < KQ; k++)
+#pragma unroll
+        for (int bi = 0; bi < B; bi++)
+#pragma unroll
+          for (int e = 0; e < NS; e++) acc[0][k][bi][e] = A(0);
+      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int, auto& xv) {
+        constexpr int NV = GE::DE * HS;  // scalars of one D block
+        if constexpr (!FUSED_D) {
+          (void)p;
+          (void)xv;
+        } else if constexpr (NV <= 16) {
+          double v[NV];
+#pragma unroll
+          for (int bj = 0; bj < B; bj++)
+#pragma unroll
+            for (int bi = 0; bi < B; bi++) {
+              A re = A(0), im = A(0);
+#pragma unroll
+              for (int k = 0; k < KQ; k++)
+#pragma unroll
+                for (int e = 0; e < NS; e += HS) {
+                  const A ar = ai[k][bi][e], br = xv[0][bj][k][e];
+                  re = fma(ar, br, re);
+                  if constexpr (CX) {
+                    const A aim = ai[k][bi][e + 1], bim = xv[0][bj][k][e + 1];
+                    re = fma(aim, bim, re);
+                    im = fma(aim, br, im);
+                    im = fma(-ar, bim, im);
+                  }
+                }
+              const int de = bj * B + bi;  // column-major element index in the block
+              v[de * HS] = (double)re;
+              if constexpr (CX) v[de * HS + 1] = (double)im;
+            }
+          reduce_scatter<L, NV>(v, l);
+          constexpr int CNT = NV >= L ? NV / L : 1;
+          if (NV >= L || l < NV) {
+            const int base = rs_base<L, NV>(l);
+            double* d = dacc + (size_t)p * NV + base;
+#pragma unroll
+            for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], d[t]);
+          }
+        } else {
+          double own[GE::DOWN][HS];
+#pragma unroll
+          for (int bj = 0; bj < B; bj++)
+#pragma unroll
+            for (int bi = 0; bi < B; bi++) {
+              A re = A(0), im = A(0);
+#pragma unroll
+              for (int k = 0; k < KQ; k++)
+#pragma unroll
+                for (int e = 0; e < NS; e += HS) {
+                  const A ar = ai[k][bi][e], br = xv[0][bj][k][e];
+                  re = fma(ar, br, re);
+                  if constexpr (CX) {
+                    const A aim = ai[k][bi][e + 1], bim = xv[0][bj][k][e + 1];
+                    re = fma(aim, bim, re);
+                    im = fma(aim, br, im);
+                    im = fma(-ar, bim, im);
+                  }
+                }
+              const double sre = group_sum<L>((double)re);
+              const double sim = CX ? group_sum<L>((double)im) : 0.0;
+              const int de = bj * B + bi;
+              if (l == de % L) {
+                own[de / L][0] = sre;
+                if constexpr (CX) own[de / L][HS - 1] = sim;
+              }
+            }
+#pragma unroll
+          for (int o = 0; o < GE::DOWN; o++) {
+            const int de = o * L + l;
+            if (de < GE::DE) {
+              double* d = dacc + ((size_t)p * GE::DE + de) * HS;
+#pragma unroll
+              for (int c = 0; c < HS; c++) d[c] = fma(fD, own[o][c], d[c]);
+            }
+          }
+        }
+      });
+#pragma unroll
+      for (int bi = 0; bi < B; bi++) {
+#pragma unroll
+        for (int k = 0; k < KQ; k++) {
+          const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
+          A a1[NS], an[NS], bo[NS], b2v[NS];
+          ldv_nt<S, NS>(am1 + off, a1, nt & NT_LOAD);
+          ldv_nt<S, NS>(amm + off, an, nt & NT_LOAD);
+          if (bm2 != nullptr) {
+            ldv_nt<S, NS>(bm2 + off, b2v, nt & NT_LOAD);
+          } else {
+#pragma unroll
+            for (int e = 0; e < NS; e++) b2v[e] = asb2 * a1[e];
+          }
+#pragma unroll
+          for (int e = 0; e < NS; e++) {
+            A v = fma(ab2, acc[0][k][bi][e], -b2v[e]);
+            v = fma(ac4, ai[k][bi][e], v);
+            v = fma(ac2, a1[e], v);
+            v = fma(aw, an[e], v);
+            bo[e] = v;
+          }
+          stv_nt<S, NS>(b_out + off, bo, nt & NT_STORE);
+        }
+      }
+    }
+  }
+}
+
+template <typename S, bool CX, int B, int L, int KQ, bool FUSED_D>
+__global__ __launch_bounds__(WG) void k_step_autodiff_stored(
+    const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
+    const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
+    const S* __restrict__ bm2, const S* __restrict__ amm, S* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  step_autodiff_stored_dev<S, CX, B, L, KQ, FUSED_D>(wg_self(), rp, ci, hv, am, am1, bm1, bm2,
+                                                     amm, b_out, dacc, nb, qp, chunk, c4, c2, w,
+                                                     sb, sb2, dscale, nt);
+}
+
 template <typename S, bool CX, int B, int L, int KQ, bool FUSED_D>
 __global__ __launch_bounds__(WG) void k_step_autodiff(
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,

@@ -281,6 +281,19 @@ class HipEngine:
         call("fkpm_moments2", self._h, int(M), *args, int(col_chunk), _dptr(out))
         return out.view(np.complex128).reshape(M, M)
 
+    def set_store_alpha(self, mode) -> None:
+        """Keep every forward vector for autodiff_matrix (FKPM_OPT_STORE_ALPHA):
+        True / 2 whenever the table fits, 1 automatic (default), False / 0
+        recompute alpha backward."""
+        call("fkpm_set_option", self._h, _capi.OPT_STORE_ALPHA,
+             2 if mode is True else 0 if mode is False else int(mode))
+
+    def alpha_table(self) -> bool:
+        """Whether the last moments(M) kept its forward vectors."""
+        v = C.c_int()
+        call("fkpm_get_alpha_table", self._h, C.byref(v))
+        return bool(v.value)
+
     def set_sweep(self, sweep: int) -> None:
         """Row sweep of every kernel kind: 0 per XCD, 1 one front, -1 the
         configuration's own (FKPM_OPT_SWEEP)."""

@@ -157,6 +157,14 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 #define FKPM_OPT_AUTOTUNE 2     /* value 1: tune the kernel geometry at first use of a shape */
 #define FKPM_OPT_MFMA 3         /* value 1 (default): matrix-core (MFMA) kernels where the block
                                    shape has them (complex 8x8 blocks); 0: vector-unit kernels */
+#define FKPM_OPT_STORE_ALPHA 5  /* value 2: fkpm_moments keeps every alpha_k when the table
+                                   fits half the free device memory (or the byte count in the
+                                   environment variable FKPM_ALPHA_TABLE_BYTES), and
+                                   fkpm_autodiff_matrix then reads alpha_{m-1} instead of
+                                   recomputing it (one gathered vector per step instead of two);
+                                   1 (default): the same when three vectors exceed the 256 MB
+                                   last-level cache; 0: the three-vector ring and the
+                                   recomputation */
 #define FKPM_OPT_SWEEP 4        /* value 0: every kernel kind sweeps the rows per XCD; s in
                                    1..64: as one front in segments of s passes per XCD
                                    (fkpm_get_kernel_sweep); -1 (default): the configuration's
@@ -193,6 +201,10 @@ int fkpm_get_kernel_mfma(fkpm_handle* h, int kind, int* mfma);
  * stay in the shared last-level cache), the rows dealt to the XCDs in
  * segments of s workgroup passes.  The autotuner times 0, 1, 2 and 8. */
 int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep);
+
+/* *stored = 1 when the last fkpm_moments kept its forward vectors for the
+ * next fkpm_autodiff_matrix (FKPM_OPT_STORE_ALPHA). */
+int fkpm_get_alpha_table(fkpm_handle* h, int* stored);
 
 /* r_compact: R has at most one nonzero per row (probing colours) and the
  * Clenshaw source term reads the compact form; tuned: the geometry came from

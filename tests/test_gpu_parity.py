@@ -216,6 +216,31 @@ def test_one_front_sweep_matches_oracle(oracle, shape, b, ob, mfma, sweep):
     assert rel_err(D.val, orc.stoch_matrix(c, D)) < FP64_TOL
 
 
+@pytest.mark.parametrize("store", [True, False])
+@pytest.mark.parametrize("name,shape,b,J,W,ob,periods,real", [CASES[2], CASES[3], CASES[4]],
+                         ids=[CASES[2][0], CASES[3][0], CASES[4][0]])
+def test_autodiff_alpha_table_and_recompute(oracle, store, name, shape, b, J, W, ob, periods,
+                                            real):
+    """autodiff_matrix with the forward vectors kept (alpha_{m-1} read) and
+    with alpha recomputed backward, on H's pattern and on a foreign one."""
+    from fastkpm_amd import lattice
+    H, R = _case(oracle, shape, b, seed=19, J=J, W=W, orbital_blocks=ob, periods=periods, real=real)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    lo, hi = lam[0] - 0.1, lam[-1] + 0.1
+    M = 48
+    eng, orc, D = _run_pair(oracle, H, R, lo, hi, M)
+    eng.set_store_alpha(store)
+    c = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
+    nb = H.n_rows
+    Dd = lattice.coo_to_bsr(nb, b, list(range(nb)), list(range(nb)), np.zeros((nb, b, b)))
+    for DD in (D, Dd):
+        mu = eng.moments(M)
+        assert eng.alpha_table() is store
+        assert rel_err(mu, orc.moments(M)) < FP64_TOL
+        eng.autodiff_matrix(c, DD)
+        assert rel_err(DD.val, orc.autodiff_matrix(c, DD)) < FP64_TOL, (name, store)
+
+
 def test_option_refusals():
     from fastkpm_amd._capi import FkpmError, OPT_SWEEP, OPT_MFMA, call
     eng = _engine()
