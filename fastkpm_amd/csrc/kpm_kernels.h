@@ -319,27 +319,59 @@ __device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step, WgId wi
   return rr;
 }
 
+// Block row offsets rp[i], rp[i+1] (zeros past r1).  The row loops load the
+// next row's pair before working on the current row.
+struct RowPtr {
+  int p0, p1;
+};
+
+__device__ __forceinline__ RowPtr row_ptr(const int* __restrict__ rp, int i, int r1) {
+  return i < r1 ? RowPtr{rp[i], rp[i + 1]} : RowPtr{0, 0};
+}
+
+// A group's first row; with one group per wave (L = 64) the index is made
+// wave-uniform, so that the row's offsets, column indices and H values are
+// scalar loads.
+template <int L>
+__device__ __forceinline__ int first_row(int i) {
+  if constexpr (L == WAVE) return __builtin_amdgcn_readfirstlane(i);
+  return i;
+}
+
 // Block row i's gather: acc[x][k][bi] += sum_p Hs_p(bi, bj) X_x[B j_p + bj]
 // over the blocks p0..p1 of the row, for NX vectors sharing the pattern, then
-// blk(p, j_p, xv) with the gathered packets xv[x][bj][k][.] of block p.  qo is the
-// scalar offset of this lane's first packet in a row.
+// blk(p, j_p, xv, pv) with the gathered packets xv[x][bj][k][.] of block p and
+// pv = pre(p), a value pre() loads for block p while block p's packets are in
+// flight (the D entries a fused outer product updates), so that its latency
+// overlaps the gather's.  qo is the scalar offset of this lane's first packet
+// in a row.  The column indices of the next batch are loaded with the current
+// batch's packets.
 // With OWN, the packets of X[0] in the row's own block (column iown) are taken
 // from `own` (already in registers) instead of being loaded again.
-template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, bool OWN, class F>
-__device__ __forceinline__ void gather_row_impl(const int* __restrict__ ci,
-                                                const S* __restrict__ hv, const S* const (&X)[NX],
-                                                size_t rs, size_t qo, int p0, int p1,
-                                                A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk,
-                                                int iown, const A (&own)[KQ][B][Vec16<S>::N]) {
+struct NoPre {};
+
+template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, bool OWN, class F,
+          class P>
+__device__ __forceinline__ void gather_row_pre(const int* __restrict__ ci,
+                                               const S* __restrict__ hv, const S* const (&X)[NX],
+                                               size_t rs, size_t qo, int p0, int p1,
+                                               A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk,
+                                               P&& pre, int iown,
+                                               const A (&own)[KQ][B][Vec16<S>::N]) {
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS, HS = GE::HS, BBS = GE::BBS;
   constexpr int U = GE::template batch<NX>();
+  using PV = decltype(pre(0));
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) jn[u] = (p0 + u < p1) ? ci[p0 + u] : -1;
   for (int p = p0; p < p1; p += U) {
     int j[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) j[u] = (p + u < p1) ? ci[p + u] : -1;
+    for (int u = 0; u < U; u++) j[u] = jn[u];
     A xv[U][NX][B][KQ][NS];
     A hh[U][BBS];
+    PV pv[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (j[u] >= 0) {
@@ -363,6 +395,7 @@ __device__ __forceinline__ void gather_row_impl(const int* __restrict__ ci,
         }
 #pragma unroll
         for (int t = 0; t < BBS; t++) hh[u][t] = (A)hv[(size_t)(p + u) * BBS + t];
+        pv[u] = pre(p + u);
       } else {
         // defined zeros past the row's end keep the batch's registers from
         // staying live across the (skipped) arithmetic
@@ -376,8 +409,11 @@ __device__ __forceinline__ void gather_row_impl(const int* __restrict__ ci,
               for (int e = 0; e < NS; e++) xv[u][x][bj][k][e] = A(0);
 #pragma unroll
         for (int t = 0; t < BBS; t++) hh[u][t] = A(0);
+        pv[u] = PV{};
       }
     }
+#pragma unroll
+    for (int u = 0; u < U; u++) jn[u] = (p + U + u < p1) ? ci[p + U + u] : -1;
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (j[u] < 0) continue;
@@ -391,9 +427,20 @@ __device__ __forceinline__ void gather_row_impl(const int* __restrict__ ci,
             for (int bi = 0; bi < B; bi++)
               cmac<CX, NS>(acc[x][k][bi], hh[u][(bj * B + bi) * HS],
                            CX ? hh[u][(bj * B + bi) * HS + HS - 1] : A(0), xv[u][x][bj][k]);
-      blk(p + u, j[u], xv[u]);
+      blk(p + u, j[u], xv[u], pv[u]);
     }
   }
+}
+
+template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, bool OWN, class F>
+__device__ __forceinline__ void gather_row_impl(const int* __restrict__ ci,
+                                                const S* __restrict__ hv, const S* const (&X)[NX],
+                                                size_t rs, size_t qo, int p0, int p1,
+                                                A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk,
+                                                int iown, const A (&own)[KQ][B][Vec16<S>::N]) {
+  gather_row_pre<S, A, CX, B, L, KQ, NX, OWN>(
+      ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int jb, auto& xv, NoPre) { blk(p, jb, xv); },
+      [](int) { return NoPre{}; }, iown, own);
 }
 
 template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, class F>
@@ -416,6 +463,23 @@ __device__ __forceinline__ void load_rows(const S* __restrict__ X, int i, size_t
       ldv<S, NS>(X + (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS, v[k][bi]);
 }
 
+// The same with the nontemporal hint chosen once for all packets, so that
+// every load of the row is issued before the first use.
+template <typename S, typename A, int B, int L, int KQ>
+__device__ __forceinline__ void load_rows_nt(const S* __restrict__ X, int i, size_t rs, size_t qo,
+                                             A (&v)[KQ][B][Vec16<S>::N], bool nt) {
+  constexpr int NS = Vec16<S>::N;
+  if (nt) {
+#pragma unroll
+    for (int bi = 0; bi < B; bi++)
+#pragma unroll
+      for (int k = 0; k < KQ; k++)
+        ldv_nt<S, NS>(X + (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS, v[k][bi], true);
+  } else {
+    load_rows<S, A, B, L, KQ>(X, i, rs, qo, v);
+  }
+}
+
 // Optional source term cr * R_i of a Chebyshev step (the Clenshaw form of
 // stoch_matrix): R dense (rd, the N x Sp probe block) or compact (one nonzero
 // per orbital row: column rcol[r] (-1 for none) with value rval[r]).
@@ -427,27 +491,47 @@ struct RSource {
   double cr;
 };
 
-// y[e] += cr * R(row, packet q) for the NS scalars of one packet.
-template <typename S, bool CX, typename A>
-__device__ __forceinline__ void add_rsource(const RSource<S>& rs_, size_t row, size_t off, int q,
-                                            A* y) {
+// y[k][bi] += cr * R(B i + bi, packet ql + k L) for all packets of a lane in
+// block row i (ql = q0 + l), every load issued before the first use.
+template <typename S, bool CX, typename A, int B, int L, int KQ>
+__device__ __forceinline__ void add_rsource_rows(const RSource<S>& rs_, int i, size_t rs,
+                                                 size_t qo, int ql,
+                                                 A (&y)[KQ][B][Vec16<S>::N]) {
   constexpr int NS = Vec16<S>::N, HS = CX ? 2 : 1, NE = NS / HS;
   const A cr = (A)rs_.cr;
   if (rs_.rd != nullptr) {
-    A r[NS];
-    ldv<S, NS>(rs_.rd + off, r);
+    A r[KQ][B][NS];
+    load_rows<S, A, B, L, KQ>(rs_.rd, i, rs, qo, r);
 #pragma unroll
-    for (int e = 0; e < NS; e++) y[e] = fma(cr, r[e], y[e]);
+    for (int k = 0; k < KQ; k++)
+#pragma unroll
+      for (int bi = 0; bi < B; bi++)
+#pragma unroll
+        for (int e = 0; e < NS; e++) y[k][bi][e] = fma(cr, r[k][bi][e], y[k][bi][e]);
   } else {
-    const int c = rs_.rcol[row] - q * NE;  // element of this packet holding the nonzero
-    if (c >= 0 && c < NE) {
+    int col[B];
+    A val[B][HS];
 #pragma unroll
-      for (int e = 0; e < NE; e++)
-        if (e == c) {
+    for (int bi = 0; bi < B; bi++) {
+      const size_t row = (size_t)B * i + bi;
+      col[bi] = rs_.rcol[row];
 #pragma unroll
-          for (int h = 0; h < HS; h++) y[e * HS + h] = fma(cr, (A)rs_.rval[row * HS + h], y[e * HS + h]);
-        }
+      for (int h = 0; h < HS; h++) val[bi][h] = (A)rs_.rval[row * HS + h];
     }
+#pragma unroll
+    for (int k = 0; k < KQ; k++)
+#pragma unroll
+      for (int bi = 0; bi < B; bi++) {
+        const int c = col[bi] - (ql + k * L) * NE;  // element of this packet holding the nonzero
+        // selects rather than a branch on c, which the compiler would turn
+        // into a dynamic index into y (and y into scratch memory)
+#pragma unroll
+        for (int e = 0; e < NE; e++)
+#pragma unroll
+          for (int h = 0; h < HS; h++)
+            y[k][bi][e * HS + h] =
+                fma(cr, e == c ? val[bi][h] : A(0), y[k][bi][e * HS + h]);
+      }
   }
 }
 
@@ -477,11 +561,14 @@ __device__ __forceinline__ void step_moments_dev(
   double dmm = 0.0, dm1 = 0.0;
   const S* const X[1] = {x};
 
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
+    const RowPtr cur = nxt;
+    nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
-    const int p0 = rp[i], p1 = rp[i + 1];
+    const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = 0; q0 < qp; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A xd[KQ][B][NS];
@@ -508,28 +595,36 @@ __device__ __forceinline__ void step_moments_dev(
                                                for (int e = 0; e < NS; e++)
                                                  xd[k][bi][e] = xv[0][bi][k][e];
                                          });
+      A out[KQ][B][NS];
+      if (z != nullptr) {
+        load_rows_nt<S, A, B, L, KQ>(z, i, rs, qo, out, nt & NT_LOAD);
+#pragma unroll
+        for (int k = 0; k < KQ; k++)
+#pragma unroll
+          for (int bi = 0; bi < B; bi++)
+#pragma unroll
+            for (int e = 0; e < NS; e++)
+              out[k][bi][e] = fma(acx, acc[0][k][bi][e], acz * out[k][bi][e]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < KQ; k++)
+#pragma unroll
+          for (int bi = 0; bi < B; bi++)
+#pragma unroll
+            for (int e = 0; e < NS; e++) out[k][bi][e] = acx * acc[0][k][bi][e];
+      }
+      if (rsrc.cr != 0.0) add_rsource_rows<S, CX, A, B, L, KQ>(rsrc, i, rs, qo, q0 + l, out);
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
 #pragma unroll
         for (int k = 0; k < KQ; k++) {
           const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
-          A zv[NS], out[NS];
-          if (z != nullptr) {
-            ldv_nt<S, NS>(z + off, zv, nt & NT_LOAD);
-          } else {
-#pragma unroll
-            for (int e = 0; e < NS; e++) zv[e] = A(0);
-          }
-#pragma unroll
-          for (int e = 0; e < NS; e++) out[e] = fma(acx, acc[0][k][bi][e], acz * zv[e]);
-          if (rsrc.cr != 0.0)
-            add_rsource<S, CX>(rsrc, (size_t)(B * i + bi), off, q0 + k * L + l, out);
-          stv_nt<S, NS>(y + off, out, nt & NT_STORE);
+          stv_nt<S, NS>(y + off, out[k][bi], nt & NT_STORE);
 #pragma unroll
           for (int e = 0; e < NS; e++) {
             const double xe = (double)xd[k][bi][e];
             dmm = fma(xe, xe, dmm);
-            dm1 = fma((double)out[e], xe, dm1);
+            dm1 = fma((double)out[k][bi][e], xe, dm1);
           }
         }
       }
@@ -568,11 +663,14 @@ __global__ __launch_bounds__(WG) void k_step_xi(
   const A acx = (A)cx, acz = (A)cz;
   const S* const X[1] = {x};
 
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
+    const RowPtr cur = nxt;
+    nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
-    const int p0 = rp[i], p1 = rp[i + 1];
+    const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = 0; q0 < qp; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A acc[1][KQ][B][NS];
@@ -583,36 +681,101 @@ __global__ __launch_bounds__(WG) void k_step_xi(
 #pragma unroll
           for (int e = 0; e < NS; e++) acc[0][k][bi][e] = A(0);
       gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [](int, int, auto&) {});
+      // own rows of z and xi (x at the first step), all packets in one batch
+      A zv[KQ][B][NS];
+      double xa[KQ][B][NS];
+      if (z != nullptr) {
+        load_rows_nt<S, A, B, L, KQ>(z, i, rs, qo, zv, nt & NT_LOAD);
+      } else {
+#pragma unroll
+        for (int k = 0; k < KQ; k++)
+#pragma unroll
+          for (int bi = 0; bi < B; bi++)
+#pragma unroll
+            for (int e = 0; e < NS; e++) zv[k][bi][e] = A(0);
+      }
+      A xo[KQ][B][NS];
+      if (first) {
+        load_rows<S, A, B, L, KQ>(x, i, rs, qo, xo);
+      } else {
+        const bool ntl = nt & NT_LOAD;
+#pragma unroll
+        for (int bi = 0; bi < B; bi++)
+#pragma unroll
+          for (int k = 0; k < KQ; k++)
+            ldv_nt<double, NS>(xi + (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS,
+                               xa[k][bi], ntl);
+      }
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
 #pragma unroll
         for (int k = 0; k < KQ; k++) {
           const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
-          A zv[NS], out[NS];
-          if (z != nullptr) {
-            ldv_nt<S, NS>(z + off, zv, nt & NT_LOAD);
-          } else {
+          A out[NS];
 #pragma unroll
-            for (int e = 0; e < NS; e++) zv[e] = A(0);
-          }
-#pragma unroll
-          for (int e = 0; e < NS; e++) out[e] = fma(acx, acc[0][k][bi][e], acz * zv[e]);
+          for (int e = 0; e < NS; e++) out[e] = fma(acx, acc[0][k][bi][e], acz * zv[k][bi][e]);
           stv_nt<S, NS>(y + off, out, nt & NT_STORE);
-          double xa[NS];
+          double xn[NS];
           if (first) {
-            A xo[NS];
-            ldv<S, NS>(x + off, xo);
 #pragma unroll
-            for (int e = 0; e < NS; e++) xa[e] = fma(c0, (double)xo[e], c1 * (double)out[e]);
+            for (int e = 0; e < NS; e++)
+              xn[e] = fma(c0, (double)xo[k][bi][e], c1 * (double)out[e]);
           } else {
-            ldv_nt<double, NS>(xi + off, xa, nt & NT_LOAD);
 #pragma unroll
-            for (int e = 0; e < NS; e++) xa[e] = fma(c1, (double)out[e], xa[e]);
+            for (int e = 0; e < NS; e++) xn[e] = fma(c1, (double)out[e], xa[k][bi][e]);
           }
-          stv_nt<double, NS>(xi + off, xa, nt & NT_STORE);
+          stv_nt<double, NS>(xi + off, xn, nt & NT_STORE);
         }
       }
     }
+  }
+}
+
+// The D entries one lane updates in a fused outer-product block p (the
+// layouts of the D updates in the autodiff steps): with NV = B*B*HS <= 16
+// scalars per block, the CNT entries at rs_base after the reduce-scatter;
+// otherwise own element o*L + l of the block (o < DOWN), HS scalars each.
+template <int N>
+struct DVals {
+  double v[N];
+};
+
+// Entry t of the prefetched D values, or *d when nothing was prefetched.
+template <int N>
+__device__ __forceinline__ double dpre_get(const DVals<N>& dp, int t, const double* d) {
+  return dp.v[t];
+}
+__device__ __forceinline__ double dpre_get(const NoPre&, int, const double* d) { return *d; }
+
+template <typename S, bool CX, int B, int L, int KQ, bool FUSED_D>
+__device__ __forceinline__ auto load_dpre(const double* __restrict__ dacc, int p, int l) {
+  using GE = Geo<S, CX, B, L, KQ>;
+  constexpr int HS = GE::HS;
+  constexpr int NV = GE::DE * HS;
+  if constexpr (!FUSED_D || (NV > 16 && GE::DOWN * HS > 8)) {
+    // large blocks (b = 8 on the vector units): the entries would hold too
+    // many registers across the gather; the update reads them itself
+    return NoPre{};
+  } else if constexpr (NV <= 16) {
+    constexpr int CNT = NV >= L ? NV / L : 1;
+    DVals<CNT> d{};
+    if (NV >= L || l < NV) {
+      const double* src = dacc + (size_t)p * NV + rs_base<L, NV>(l);
+#pragma unroll
+      for (int t = 0; t < CNT; t++) d.v[t] = src[t];
+    }
+    return d;
+  } else {
+    DVals<GE::DOWN * HS> d{};
+#pragma unroll
+    for (int o = 0; o < GE::DOWN; o++) {
+      const int de = o * L + l;
+      if (de < GE::DE) {
+#pragma unroll
+        for (int c = 0; c < HS; c++) d.v[o * HS + c] = dacc[((size_t)p * GE::DE + de) * HS + c];
+      }
+    }
+    return d;
   }
 }
 
@@ -649,12 +812,16 @@ __device__ __forceinline__ void step_autodiff_dev(
   const S* const X[2] = {am, vb1 ? am : bm1};
   const A a2 = (A)2, ab2 = (A)(2.0 * hb1), ac4 = (A)c4, ac2 = (A)c2, aw = (A)w, asb2 = (A)sb2;
   const double fD = dscale * hb1;
+  using DPre = decltype(load_dpre<S, CX, B, L, KQ, FUSED_D>(dacc, 0, 0));
 
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
+    const RowPtr cur = nxt;
+    nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
-    const int p0 = rp[i], p1 = rp[i + 1];
+    const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = 0; q0 < qp; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A ai[KQ][B][NS];
@@ -668,8 +835,8 @@ __device__ __forceinline__ void step_autodiff_dev(
           for (int bi = 0; bi < B; bi++)
 #pragma unroll
             for (int e = 0; e < NS; e++) acc[x][k][bi][e] = A(0);
-      gather_row_impl<S, A, CX, B, L, KQ, 2, true>(
-          ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int, auto& xv) {
+      gather_row_pre<S, A, CX, B, L, KQ, 2, true>(
+          ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int, auto& xv, const DPre& dp) {
             constexpr int NV = GE::DE * HS;  // scalars of one D block
             if constexpr (FUSED_D && NV <= 16) {
               // D_p(bi, bj) += fD * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s]):
@@ -703,7 +870,7 @@ __device__ __forceinline__ void step_autodiff_dev(
                 const int base = rs_base<L, NV>(l);
                 double* d = dacc + (size_t)p * NV + base;
 #pragma unroll
-                for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], d[t]);
+                for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], dpre_get(dp, t, d + t));
               }
             } else if constexpr (FUSED_D) {
               double own[GE::DOWN][HS];
@@ -739,33 +906,39 @@ __device__ __forceinline__ void step_autodiff_dev(
                 if (de < GE::DE) {
                   double* d = dacc + ((size_t)p * GE::DE + de) * HS;
 #pragma unroll
-                  for (int c = 0; c < HS; c++) d[c] = fma(fD, own[o][c], d[c]);
+                  for (int c = 0; c < HS; c++) d[c] = fma(fD, own[o][c], dpre_get(dp, o * HS + c, d + c));
                 }
               }
             }
           },
-          i, ai);
+          [&](int p) { return load_dpre<S, CX, B, L, KQ, FUSED_D>(dacc, p, l); }, i, ai);
+      // own rows of alpha_{m+1} and beta_{m+2}, all packets in one batch
+      A a1[KQ][B][NS], b2v[KQ][B][NS];
+      load_rows_nt<S, A, B, L, KQ>(am1, i, rs, qo, a1, nt & NT_LOAD);
+      if (bm2 != nullptr) {
+        load_rows_nt<S, A, B, L, KQ>(bm2, i, rs, qo, b2v, nt & NT_LOAD);
+      } else {
+#pragma unroll
+        for (int k = 0; k < KQ; k++)
+#pragma unroll
+          for (int bi = 0; bi < B; bi++)
+#pragma unroll
+            for (int e = 0; e < NS; e++) b2v[k][bi][e] = asb2 * a1[k][bi][e];
+      }
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
 #pragma unroll
         for (int k = 0; k < KQ; k++) {
           const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
-          A a1[NS], an[NS], bo[NS], b2v[NS];
-          ldv_nt<S, NS>(am1 + off, a1, nt & NT_LOAD);
+          A an[NS], bo[NS];
 #pragma unroll
-          for (int e = 0; e < NS; e++) an[e] = fma(a2, acc[0][k][bi][e], -a1[e]);
+          for (int e = 0; e < NS; e++) an[e] = fma(a2, acc[0][k][bi][e], -a1[k][bi][e]);
           stv_nt<S, NS>(a_out + off, an, nt & NT_STORE);
-          if (bm2 != nullptr) {
-            ldv_nt<S, NS>(bm2 + off, b2v, nt & NT_LOAD);
-          } else {
-#pragma unroll
-            for (int e = 0; e < NS; e++) b2v[e] = asb2 * a1[e];
-          }
 #pragma unroll
           for (int e = 0; e < NS; e++) {
-            A v = fma(ab2, acc[1][k][bi][e], -b2v[e]);
+            A v = fma(ab2, acc[1][k][bi][e], -b2v[k][bi][e]);
             v = fma(ac4, ai[k][bi][e], v);
-            v = fma(ac2, a1[e], v);
+            v = fma(ac2, a1[k][bi][e], v);
             v = fma(aw, an[e], v);
             bo[e] = v;
           }
@@ -811,12 +984,17 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
   const S* const X[1] = {vb1 ? am : bm1};
   const A ab2 = (A)(2.0 * hb1), ac4 = (A)c4, ac2 = (A)c2, aw = (A)w, asb2 = (A)sb2;
   const double fD = dscale * hb1;
+  using DPre = decltype(load_dpre<S, CX, B, L, KQ, FUSED_D>(dacc, 0, 0));
+  const A none[KQ][B][NS] = {};
 
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
+    const RowPtr cur = nxt;
+    nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
-    const int p0 = rp[i], p1 = rp[i + 1];
+    const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = 0; q0 < qp; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A ai[KQ][B][NS];
@@ -828,7 +1006,7 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
         for (int bi = 0; bi < B; bi++)
 #pragma unroll
           for (int e = 0; e < NS; e++) acc[0][k][bi][e] = A(0);
-      gather_row<S, A, CX, B, L, KQ, 1>(ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int, auto& xv) {
+      gather_row_pre<S, A, CX, B, L, KQ, 1, false>(ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int, auto& xv, const DPre& dp) {
         constexpr int NV = GE::DE * HS;  // scalars of one D block
         if constexpr (!FUSED_D) {
           (void)p;
@@ -863,7 +1041,7 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
             const int base = rs_base<L, NV>(l);
             double* d = dacc + (size_t)p * NV + base;
 #pragma unroll
-            for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], d[t]);
+            for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], dpre_get(dp, t, d + t));
           }
         } else {
           double own[GE::DOWN][HS];
@@ -899,31 +1077,37 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
             if (de < GE::DE) {
               double* d = dacc + ((size_t)p * GE::DE + de) * HS;
 #pragma unroll
-              for (int c = 0; c < HS; c++) d[c] = fma(fD, own[o][c], d[c]);
+              for (int c = 0; c < HS; c++) d[c] = fma(fD, own[o][c], dpre_get(dp, o * HS + c, d + c));
             }
           }
         }
-      });
+      }, [&](int p) { return load_dpre<S, CX, B, L, KQ, FUSED_D>(dacc, p, l); }, -1, none);
+      // own rows of alpha_{m+1}, alpha_{m-1} and beta_{m+2}, all packets in one batch
+      A a1[KQ][B][NS], an[KQ][B][NS], b2v[KQ][B][NS];
+      load_rows_nt<S, A, B, L, KQ>(am1, i, rs, qo, a1, nt & NT_LOAD);
+      load_rows_nt<S, A, B, L, KQ>(amm, i, rs, qo, an, nt & NT_LOAD);
+      if (bm2 != nullptr) {
+        load_rows_nt<S, A, B, L, KQ>(bm2, i, rs, qo, b2v, nt & NT_LOAD);
+      } else {
+#pragma unroll
+        for (int k = 0; k < KQ; k++)
+#pragma unroll
+          for (int bi = 0; bi < B; bi++)
+#pragma unroll
+            for (int e = 0; e < NS; e++) b2v[k][bi][e] = asb2 * a1[k][bi][e];
+      }
 #pragma unroll
       for (int bi = 0; bi < B; bi++) {
 #pragma unroll
         for (int k = 0; k < KQ; k++) {
           const size_t off = (size_t)(B * i + bi) * rs + qo + (size_t)k * L * NS;
-          A a1[NS], an[NS], bo[NS], b2v[NS];
-          ldv_nt<S, NS>(am1 + off, a1, nt & NT_LOAD);
-          ldv_nt<S, NS>(amm + off, an, nt & NT_LOAD);
-          if (bm2 != nullptr) {
-            ldv_nt<S, NS>(bm2 + off, b2v, nt & NT_LOAD);
-          } else {
-#pragma unroll
-            for (int e = 0; e < NS; e++) b2v[e] = asb2 * a1[e];
-          }
+          A bo[NS];
 #pragma unroll
           for (int e = 0; e < NS; e++) {
-            A v = fma(ab2, acc[0][k][bi][e], -b2v[e]);
+            A v = fma(ab2, acc[0][k][bi][e], -b2v[k][bi][e]);
             v = fma(ac4, ai[k][bi][e], v);
-            v = fma(ac2, a1[e], v);
-            v = fma(aw, an[e], v);
+            v = fma(ac2, a1[k][bi][e], v);
+            v = fma(aw, an[k][bi][e], v);
             bo[e] = v;
           }
           stv_nt<S, NS>(b_out + off, bo, nt & NT_STORE);
