@@ -679,10 +679,11 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
   // keep every alpha_k for autodiff_matrix (no backward recomputation of
   // alpha) when the table fits half the free device memory (or the byte
   // count in FKPM_ALPHA_TABLE_BYTES) and the autodiff step runs on the
-  // vector units.  Automatically only when the three-vector ring would not
-  // fit the 256 MB last-level cache anyway: below that the forward steps
-  // run faster on the ring (measured: FP32 C2 forward step 37 -> 49 us on a
-  // table, autodiff 120 -> 110 us).
+  // vector units.  Automatically only when the three-vector ring exceeds
+  // the 32 MB of L2 the chip has: below that the forward steps lose more
+  // on the table's write-back than the backward steps gain.  (Measured at
+  // C2, FP32: forward step 35 -> 45 us on a table, autodiff 107 -> 91 us;
+  // FP64: 81 -> 81 us, 191 -> 167 us.)
   {
     const size_t need = (size_t)nsteps * h->vbytes();
     size_t free_b = 0, total_b = 0;
@@ -690,7 +691,7 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
     size_t budget = free_b / 2 + h->atab.bytes;
     if (const char* e = getenv("FKPM_ALPHA_TABLE_BYTES")) budget = (size_t)strtoull(e, nullptr, 10);
     const bool want = h->store_alpha == 2 ||
-                      (h->store_alpha == 1 && 3 * h->vbytes() > ((size_t)256 << 20));
+                      (h->store_alpha == 1 && 3 * h->vbytes() > ((size_t)32 << 20));
     h->a_stored = false;
     if (want && !h->kc[K_AD].mfma && need <= budget) {
       CHK(h->atab.ensure(need));

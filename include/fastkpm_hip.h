@@ -162,8 +162,8 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
                                    environment variable FKPM_ALPHA_TABLE_BYTES), and
                                    fkpm_autodiff_matrix then reads alpha_{m-1} instead of
                                    recomputing it (one gathered vector per step instead of two);
-                                   1 (default): the same when three vectors exceed the 256 MB
-                                   last-level cache; 0: the three-vector ring and the
+                                   1 (default): the same when three vectors exceed the
+                                   32 MB of L2; 0: the three-vector ring and the
                                    recomputation */
 #define FKPM_OPT_SWEEP 4        /* value 0: every kernel kind sweeps the rows per XCD; s in
                                    1..64: as one front in segments of s passes per XCD
