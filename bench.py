@@ -161,7 +161,8 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     alpha_table = eng.alpha_table()  # autodiff reads the kept forward vectors
     eng.autodiff_matrix(cE, Dg)
     kcfg = {name: eng.kernel_config(k) for k, name in
-            ((0, "moments"), (1, "xi"), (2, "autodiff"), (3, "outer"))}
+            ((0, "moments"), (1, "xi"), (2, "autodiff"), (3, "outer"), (4, "moments_table"),
+             (5, "autodiff_table"))}
     return dict(eng=eng, es=es, N=N, S=S, M=M, mu=mu, energy=energy, wall=wall, times=times,
                 kernel_ms=tot, Dg=Dg.val.copy(), Ds=Ds.val.copy(), config=eng.config(),
                 kernel_config=kcfg, tune_s=t_tune, state=eng.state(), alpha_table=alpha_table,
@@ -449,7 +450,12 @@ def main(argv=None) -> int:
                    "frac": round(gbs / HBM_PEAK_GBS, 4),
                    "ms_per_step": round(ms / args.steps, 3)}
     dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
-    kc = r["kernel_config"]["autodiff" if dom == "autodiff" else "moments"]
+    # the forward and backward steps run the table kinds when the forward
+    # vectors are kept; the Clenshaw steps always the ring kind
+    kind = {"moments": "moments", "autodiff": "autodiff", "stoch": "moments"}[dom]
+    if r["alpha_table"] and dom != "stoch":
+        kind += "_table"
+    kc = r["kernel_config"][kind]
     if kc.get("mfma"):
         rr = "_rr" if kc.get("mfma_variant") == 2 else ""
         names = {"moments": f"k_mfma_step{rr}", "stoch": f"k_mfma_step{rr} (Clenshaw)",

@@ -16,6 +16,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 OPT_STOCH_METHOD, OPT_AUTOTUNE, OPT_MFMA, OPT_SWEEP, OPT_STORE_ALPHA = 1, 2, 3, 4, 5
 STOCH_CLENSHAW, STOCH_FORWARD = 0, 1
 KERNEL_MOMENTS, KERNEL_XI, KERNEL_AUTODIFF, KERNEL_OUTER = 0, 1, 2, 3
+KERNEL_MOMENTS_TABLE, KERNEL_AUTODIFF_TABLE = 4, 5
 
 ERRORS = {-1: "FKPM_E_ARG", -2: "FKPM_E_STATE", -3: "FKPM_E_HIP", -4: "FKPM_E_NODEV",
           -5: "FKPM_E_UNSUP"}

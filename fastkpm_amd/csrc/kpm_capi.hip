@@ -144,7 +144,10 @@ int check_pattern(const char* what, int nb, int nnzb, const int* row_ptr, const 
 }
 
 // Kernel kinds with a geometry of their own.
-enum KKind { K_MOM = 0, K_XI = 1, K_AD = 2, K_OUT = 3, NKIND = 4 };
+// K_MOMT / K_ADS: the forward step writing the table of forward vectors and
+// the backward step reading it (tuned apart: a step that writes a fresh
+// vector per launch meets the caches differently from one cycling a ring).
+enum KKind { K_MOM = 0, K_XI = 1, K_AD = 2, K_OUT = 3, K_MOMT = 4, K_ADS = 5, NKIND = 6 };
 
 // Geometry of one kernel kind: lanes per block row, packets per lane,
 // workgroup cap and nontemporal bits; nwg/chunk derived from them.
@@ -294,10 +297,10 @@ void derive_grid(int nb, KCfg& c) {
   }
 }
 
-// Matrix-core kernels exist for complex 8x8 blocks (forward/Clenshaw steps
-// and the autodiff step).
+// Matrix-core kernels exist for complex 8x8 blocks (forward/Clenshaw steps,
+// also into the table, and the autodiff step that recomputes alpha).
 bool mfma_avail(const fkpm_handle* h, int kind) {
-  return h->mfma_on && h->cx && h->b == 8 && (kind == K_MOM || kind == K_AD);
+  return h->mfma_on && h->cx && h->b == 8 && (kind == K_MOM || kind == K_MOMT || kind == K_AD);
 }
 
 // The row-resident variants: rows of at most 8 blocks, byte offsets of a
@@ -342,7 +345,7 @@ auto adp_fn(const fkd::KSet<S>& ks, const KCfg& c) {
 KCfg default_cfg(fkpm_handle* h, int kind) {
   static const int pref_gather[] = {64, 32, 16, 8, 4};
   static const int pref_reduce[] = {16, 8, 32, 4, 64};
-  const int* pref = (kind == K_MOM || kind == K_XI) ? pref_gather : pref_reduce;
+  const int* pref = (kind == K_MOM || kind == K_XI || kind == K_MOMT) ? pref_gather : pref_reduce;
   KCfg c;
   for (int t = 0; t < 5 && c.L == 0; t++) {
     const int L = pref[t];
@@ -367,9 +370,19 @@ KCfg default_cfg(fkpm_handle* h, int kind) {
   if (mfma_avail(h, kind)) {
     c.mfma = 1;
     // row-resident: the forward step always, the autodiff step in FP64
-    if (mfma_rr_avail(h, kind) && (kind == K_MOM || !h->single)) c.mfma = 2;
+    if (mfma_rr_avail(h, kind) && (kind == K_MOM || kind == K_MOMT || !h->single)) c.mfma = 2;
   }
   return c;
+}
+
+// Whether moments(M) keeps the forward vectors (when they fit): always with
+// FKPM_OPT_STORE_ALPHA 2; with 1 (automatic) when the three-vector ring
+// exceeds the 32 MB of L2: below that the forward steps lose more on the
+// table's write-back than the backward steps gain.  (Measured at C2, FP32:
+// forward step 35 -> 38 us on a table, autodiff 107 -> 91 us; FP64: 81 ->
+// 81 us, 191 -> 167 us.)
+bool table_wanted(const fkpm_handle* h) {
+  return h->store_alpha == 2 || (h->store_alpha == 1 && 3 * h->vbytes() > ((size_t)32 << 20));
 }
 
 // Process-wide cache of tuned geometries, keyed by the problem shape.
@@ -378,8 +391,8 @@ std::map<std::string, std::array<KCfg, NKIND>> g_tune_cache;
 
 std::string tune_key(const fkpm_handle* h) {
   char k[128];
-  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb,
-           h->nnzb, h->mfma_on ? 1 : 0, h->max_deg <= 8 ? 1 : 0);
+  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb,
+           h->nnzb, h->mfma_on ? 1 : 0, h->max_deg <= 8 ? 1 : 0, table_wanted(h) ? 1 : 0);
   return k;
 }
 
@@ -508,7 +521,8 @@ int ensure_ready(fkpm_handle* h) {
 }
 
 int ensure_partials(fkpm_handle* h, int nsteps) {
-  CHK(h->partial.ensure((size_t)nsteps * h->kc[K_MOM].nwg * 2 * sizeof(double)));
+  const int nwg = std::max(h->kc[K_MOM].nwg, h->kc[K_MOMT].nwg);
+  CHK(h->partial.ensure((size_t)nsteps * nwg * 2 * sizeof(double)));
   CHK(h->psum.ensure((size_t)nsteps * 2 * sizeof(double)));
   return 0;
 }
@@ -576,8 +590,39 @@ int tune_t(fkpm_handle* h) {
   static const int caps[] = {1024, 4096, 16384};
   static const int nts[] = {0, fkd::NT_LOAD | fkd::NT_STORE};
   static const int sweeps[] = {0, 1, 2, 8};
-  const int kinds[] = {K_MOM, K_XI, K_AD};
+  // The table kinds are timed on ns slots (at least 512 MB, more than the
+  // last-level cache holds) walked as the recursions walk the table: a fresh
+  // slot per forward step, the kept vectors backwards.  Without a table in
+  // prospect they take the ring kinds' choice.
+  int ns = 0;
+  int tstep = 0;
+  auto slot = [&](int k) { return (S*)((char*)h->atab.p + (size_t)k * vb); };
+  const int kinds[] = {K_MOM, K_XI, K_AD, K_MOMT, K_ADS};
   for (int kind : kinds) {
+    if (kind == K_MOMT) {
+      if (table_wanted(h) && !h->kc[K_AD].mfma) {
+        const size_t want = ((size_t)512 << 20) / vb + 1;
+        const int n = (int)std::min<size_t>(64, std::max<size_t>(8, want));
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        if ((size_t)n * vb <= free_b / 4 + h->atab.bytes) {
+          CHK(h->atab.ensure((size_t)n * vb));
+          ns = n;
+          const KCfg& c = h->kc[K_MOM];
+          const auto& ks = h->ks<S>(K_MOM);
+          for (int k = 0; k < ns; k++)  // every slot = Hs R
+            CHK(launch(mom_fn(ks, c), dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R,
+                       (const S*)nullptr, slot(k), h->nb, h->qp, c.chunk, 1.0, 0.0,
+                       (double*)nullptr, 0, no_rsource<S>()));
+        }
+      }
+      if (ns == 0) {
+        CHK(set_kind(h, K_MOMT, h->kc[K_MOM]));
+        // no table while autodiff runs on the matrix cores
+        CHK(set_kind(h, K_ADS, h->kc[K_AD].mfma ? default_cfg(h, K_ADS) : h->kc[K_AD]));
+        break;
+      }
+    }
     KCfg best = h->kc[kind];
     float best_ms = 1e30f;
     std::vector<KCfg> cands;
@@ -614,6 +659,22 @@ int tune_t(fkpm_handle* h) {
           // steps rotate their buffers as the recursions do (one continuous
           // recurrence over all candidates keeps the values bounded)
           auto run = [&]() -> int {
+            if (kind == K_MOMT) {  // alpha_{t+2} = 2 Hs alpha_{t+1} - alpha_t into a fresh slot
+              const int t = tstep++;
+              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+                            (const S*)slot((t + 1) % ns), (const S*)slot(t % ns),
+                            slot((t + 2) % ns), h->nb, h->qp, k.chunk, 2.0, -1.0,
+                            h->partial.as<double>(), k.nt, no_rsource<S>());
+            }
+            if (kind == K_ADS) {  // backward over slots ns-2 .. 1, beta in the ring
+              const int t = tstep++;
+              const int m = ns - 2 - t % (ns - 2);
+              return launch(ks.ad_stored, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+                            (const S*)slot(m), (const S*)slot(m + 1),
+                            (const S*)bring[(t + 1) % 3], (const S*)bring[t % 3],
+                            (const S*)slot(m - 1), bring[(t + 2) % 3], h->tune_dacc.as<double>(),
+                            h->nb, h->qp, k.chunk, 0.1, 0.1, 0.1, 1.0, 0.0, 2.0, k.nt);
+            }
             const int t = step++;
             S* x = ring[(t + 1) % 3];
             S* z = ring[t % 3];
@@ -677,29 +738,24 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
   const int nsteps = M / 2;
   CHK(ensure_partials(h, nsteps));
   // keep every alpha_k for autodiff_matrix (no backward recomputation of
-  // alpha) when the table fits half the free device memory (or the byte
-  // count in FKPM_ALPHA_TABLE_BYTES) and the autodiff step runs on the
-  // vector units.  Automatically only when the three-vector ring exceeds
-  // the 32 MB of L2 the chip has: below that the forward steps lose more
-  // on the table's write-back than the backward steps gain.  (Measured at
-  // C2, FP32: forward step 35 -> 45 us on a table, autodiff 107 -> 91 us;
-  // FP64: 81 -> 81 us, 191 -> 167 us.)
+  // alpha) when wanted (table_wanted), the table fits half the free device
+  // memory (or the byte count in FKPM_ALPHA_TABLE_BYTES) and the autodiff
+  // step runs on the vector units.
   {
     const size_t need = (size_t)nsteps * h->vbytes();
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     size_t budget = free_b / 2 + h->atab.bytes;
     if (const char* e = getenv("FKPM_ALPHA_TABLE_BYTES")) budget = (size_t)strtoull(e, nullptr, 10);
-    const bool want = h->store_alpha == 2 ||
-                      (h->store_alpha == 1 && 3 * h->vbytes() > ((size_t)32 << 20));
     h->a_stored = false;
-    if (want && !h->kc[K_AD].mfma && need <= budget) {
+    if (table_wanted(h) && !h->kc[K_AD].mfma && need <= budget) {
       CHK(h->atab.ensure(need));
       h->a_stored = true;
     }
   }
-  const KCfg& kc = h->kc[K_MOM];
-  const auto& ks = h->ks<S>(K_MOM);
+  const int kind = h->a_stored ? K_MOMT : K_MOM;
+  const KCfg& kc = h->kc[kind];
+  const auto& ks = h->ks<S>(kind);
   const dim3 grid(kc.nwg), block(fkd::WG);
   CHK(timing_begin(h));
   for (int m = 0; m < nsteps; m++) {
@@ -861,8 +917,10 @@ template <class S>
 int autodiff_t(fkpm_handle* h, int M, const double* c) {
   const dim3 block(fkd::WG);
   const KCfg& ka = h->kc[K_AD];
+  const KCfg& kt = h->kc[K_ADS];
   const KCfg& ko = h->kc[K_OUT];
   const auto& ksa = h->ks<S>(K_AD);
+  const auto& kst = h->ks<S>(K_ADS);
   const auto& kso = h->ks<S>(K_OUT);
   const int* rp = h->rp.as<const int>();
   const int* ci = h->ci.as<const int>();
@@ -903,9 +961,9 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
         if (!h->d_fused)  // D += 2 alpha_m beta_{m+1}^+ on D's own pattern
           CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, am, first ? am : bm1,
                      dacc, h->nb, h->qp, ko.chunk, first ? 2.0 * sb : 2.0));
-        CHK(launch(h->d_fused ? ksa.ad_stored : ksa.ad_stored_plain, dim3(ka.nwg), block,
+        CHK(launch(h->d_fused ? kst.ad_stored : kst.ad_stored_plain, dim3(kt.nwg), block,
                    h->stream, rp, ci, hv, am, am1, bm1, bm2, (const S*)aout, bout, dacc, h->nb,
-                   h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
+                   h->qp, kt.chunk, c4, c2, w, sb, sb2, 2.0, kt.nt));
       } else if (h->d_fused) {
         CHK(launch(adf_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
                    aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));

@@ -179,6 +179,9 @@ int fkpm_set_option(fkpm_handle* h, int option, int value);
 #define FKPM_KERNEL_XI 1       /* forward step with FP64 xi accumulation */
 #define FKPM_KERNEL_AUTODIFF 2 /* backward step with the fused outer product */
 #define FKPM_KERNEL_OUTER 3    /* sparse outer product on D's pattern */
+#define FKPM_KERNEL_MOMENTS_TABLE 4  /* forward step writing the kept forward vectors
+                                        (FKPM_OPT_STORE_ALPHA) */
+#define FKPM_KERNEL_AUTODIFF_TABLE 5 /* backward step reading them */
 
 /* Autotune the geometry (lanes per block row, packets per lane, workgroup
  * count, nontemporal hints) of every kernel kind on the current H and R by

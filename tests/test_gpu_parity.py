@@ -375,10 +375,13 @@ def test_every_compiled_geometry(oracle):
         eng.set_geometry(8, 1, 0)  # not compiled for b = 2
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_autotuned_engine(oracle, prec):
+@pytest.mark.parametrize("prec,store", [("fp64", 1), ("fp32", 1), ("fp64", 2), ("fp32", 2)])
+def test_autotuned_engine(oracle, prec, store):
+    """store 2: the forward vectors are kept, and the tuner also times the
+    table-writing forward step and the table-reading backward step."""
     H, R, es, orc = _pair(oracle, shape=(16, 16), J=2.0)
     eng = _engine("cx_double", prec)
+    eng.set_store_alpha(store)
     eng.set_R(R)
     eng.set_H(H, es)
     eng.autotune()
@@ -386,6 +389,7 @@ def test_autotuned_engine(oracle, prec):
     M = 120
     c = fermi_coeffs(oracle, M, es.lo, es.hi)
     mu = eng.moments(M)
+    assert eng.alpha_table() is (store == 2)
     D = H.like()
     eng.autodiff_matrix(c, D)
     S = H.like()
@@ -396,9 +400,10 @@ def test_autotuned_engine(oracle, prec):
     assert rel_err(S.val, orc.stoch_matrix(c, S)) < (tol if prec == "fp64" else 1e-4)
     # a second engine of the same shape reuses the tuned geometry
     eng2 = _engine("cx_double", prec)
+    eng2.set_store_alpha(store)
     eng2.set_R(R)
     eng2.set_H(H, es)
     eng2.moments(4)
     assert eng2.state()["tuned"]
-    for k in range(3):
+    for k in range(6):
         assert eng2.kernel_config(k) == eng.kernel_config(k)
