@@ -65,9 +65,12 @@ def build_workload(name: str, seed: int):
     return H, groups, cfg
 
 
-def algorithmic_bytes(H, N: int, S: int, v: int, vacc: int = 16):
+def algorithmic_bytes(H, N: int, S: int, v: int, vacc: int = 16, table_K: int | None = None):
     """Algorithmic HBM bytes per launch of each step kernel (S logical columns,
-    v bytes per vector element, vacc bytes per FP64 accumulator element)."""
+    v bytes per vector element, vacc bytes per FP64 accumulator element).
+    table_K: stoch_matrix ran from the kept forward vectors (one pass over
+    alpha_0 .. alpha_K, then K Chebyshev steps), so its figure is the call's
+    bytes over its K + 1 launches."""
     b = H.b_len
     nnzb = H.n_blocks()
     h_bytes = nnzb * b * b * v + 4 * nnzb + 4 * (H.n_rows + 1)
@@ -79,7 +82,10 @@ def algorithmic_bytes(H, N: int, S: int, v: int, vacc: int = 16):
         "moments": 3 * vec + h_bytes,
         # Clenshaw step b_k = c_k R + 2 H b_{k+1} - b_{k+2}: read b_{k+1}, b_{k+2},
         # write b_k, read H (R in compact form: 4 + v bytes per orbital row)
-        "stoch": 3 * vec + h_bytes + N * (4 + v),
+        "stoch": (3 * vec + h_bytes + N * (4 + v) if table_K is None else
+                  # read alpha_0..alpha_K, write u and v; K steps; read v once more
+                  ((table_K + 1) * vec + 2 * vec + table_K * (3 * vec + h_bytes) + vec)
+                  / (table_K + 1)),
         # read alpha_m, alpha_{m+1}, beta_{m+1}, beta_{m+2}; write alpha_{m-1},
         # beta_m; read H; read-modify-write D (FP64)
         "autodiff": 6 * vec + h_bytes + 2 * d_bytes,
@@ -154,6 +160,7 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     eng.synchronize()
     wall = time.perf_counter() - t_all
     eng.set_timing(False)
+    stoch_method = eng.stoch_method()  # from the kept forward vectors, or Clenshaw
     energy = float(np.dot(cE, mu))
     eng.get_D(Ds)  # the device holds the last computed D: stoch_matrix's
     # copy of the gradient D: one more moments + autodiff_matrix to the host
@@ -166,6 +173,7 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     return dict(eng=eng, es=es, N=N, S=S, M=M, mu=mu, energy=energy, wall=wall, times=times,
                 kernel_ms=tot, Dg=Dg.val.copy(), Ds=Ds.val.copy(), config=eng.config(),
                 kernel_config=kcfg, tune_s=t_tune, state=eng.state(), alpha_table=alpha_table,
+                stoch_method=stoch_method,
                 energy_scale_s=None if "es" in cfg else t_es)
 
 
@@ -438,7 +446,8 @@ def main(argv=None) -> int:
     ms_per_step = 1e3 * r["wall"] / args.steps
     value = N * M * S / (r["wall"] / args.steps)
     v = 16 if args.precision == "fp64" else 8
-    abytes = algorithmic_bytes(H, N, S, v)
+    abytes = algorithmic_bytes(H, N, S, v,
+                               table_K=(M + 1) // 2 if r["stoch_method"] == "table" else None)
     kern = {}
     for k, (ms, n) in r["kernel_ms"].items():
         if n == 0:
@@ -462,7 +471,9 @@ def main(argv=None) -> int:
                  "autodiff": f"k_mfma_autodiff{rr}"}
         geo = f"b={H.b_len}, matrix cores, wg={kc['n_wg']}"
     else:
-        names = {"moments": "k_step_moments", "stoch": "k_step_moments (Clenshaw)",
+        names = {"moments": "k_step_moments",
+                 "stoch": ("k_table_sums + k_step_moments (from the kept forward vectors)"
+                           if r["stoch_method"] == "table" else "k_step_moments (Clenshaw)"),
                  "autodiff": "k_step_autodiff_stored" if r["alpha_table"] else "k_step_autodiff"}
         geo = f"b={H.b_len}, lanes={kc['lanes']}, kq={kc['kq']}, wg={kc['n_wg']}, nt={kc['nt']}"
     roof = {"bound": "hbm", "achieved": kern[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
@@ -506,6 +517,7 @@ def main(argv=None) -> int:
         "roofline": roof,
         "kernels": kern,
         "kernel_config": r["kernel_config"],
+        "stoch_method": r["stoch_method"],
         "autotune_s": round(r["tune_s"], 3),
         "energy_scale_ms": (None if r["energy_scale_s"] is None
                             else round(1e3 * r["energy_scale_s"], 2)),

@@ -201,11 +201,15 @@ struct fkpm_handle {
   bool tuned = false;
   bool autotune_on_use = false;
   bool mfma_on = true;  // matrix-core kernels where the block shape has them
-  int stoch_method = FKPM_STOCH_CLENSHAW;
+  int stoch_method = FKPM_STOCH_AUTO;
+  int stoch_used = -1;  // method the last stoch_matrix ran (FKPM_STOCH_*)
   DevBuf R, A[3], Bv[3], xi, partial, psum, tune_dacc;
   int alpha_M = 0;  // M of the moments whose final state is held, 0 if none
   DevBuf atab;            // alpha_1 .. alpha_{M/2} of the last moments(M), when it fits
   bool a_stored = false;  // the forward vectors live in atab (else the ring A[3])
+  int tab_K = 0;          // atab holds alpha_1 .. alpha_tab_K of the current H and R
+  DevBuf tabc;            // stoch_matrix weights of the table vectors
+  std::vector<double> tabc_host;
   int store_alpha = 1;    // FKPM_OPT_STORE_ALPHA: 0 never, 1 auto, 2 whenever the table fits
   int sweep_force = -1;  // FKPM_OPT_SWEEP: row sweep of every kind (-1: per configuration)
 
@@ -516,6 +520,7 @@ int ensure_ready(fkpm_handle* h) {
   // A-buffers: every element is written by the recursion before it is read.
   h->r_dirty = false;
   h->alpha_M = 0;
+  h->tab_K = 0;
   if (h->autotune_on_use && !h->tuned && h->force_L == 0) CHK(tune(h));
   return 0;
 }
@@ -720,6 +725,7 @@ int tune_t(fkpm_handle* h) {
   outc.nt = 0;
   CHK(set_kind(h, K_OUT, outc));
   h->alpha_M = 0;
+  h->tab_K = 0;  // the table slots held tuning data
   h->tuned = true;
   std::array<KCfg, NKIND> cc;
   for (int k = 0; k < NKIND; k++) cc[k] = h->kc[k];
@@ -749,6 +755,7 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
     if (const char* e = getenv("FKPM_ALPHA_TABLE_BYTES")) budget = (size_t)strtoull(e, nullptr, 10);
     h->a_stored = false;
     if (table_wanted(h) && !h->kc[K_AD].mfma && need <= budget) {
+      h->tab_K = 0;  // rewritten below (and lost if the table grows)
       CHK(h->atab.ensure(need));
       h->a_stored = true;
     }
@@ -769,6 +776,7 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
                kc.nt, no_rsource<S>()));
   }
   CHK(timing_end(h, &h->ms_mom, &h->n_mom, nsteps));
+  if (h->a_stored) h->tab_K = nsteps;
   CHK(launch(&fkd::k_reduce_partials, dim3(nsteps), block, h->stream,
              (const double*)h->partial.as<double>(), kc.nwg, h->psum.as<double>()));
   std::vector<double> ps((size_t)nsteps * 2);
@@ -855,6 +863,63 @@ int stoch_clenshaw(fkpm_handle* h, int M, const double* c, const S** xi_out) {
   return launches + 1;
 }
 
+// xi from the kept forward vectors alpha_0 .. alpha_K (K = ceil(M/2), coefficients
+// padded with zeros to 2K): by T_{K+j} = 2 T_K T_j - T_{K-j},
+//   xi = sum_{m<2K} c_m T_m(Hs) R = v + 2 T_K(Hs) u,
+//   v = sum_{i=0..K} ([i<K] c_i - [i>=1] c_{2K-i}) alpha_i,
+//   u = sum_{i<K} c_{K+i} alpha_i,
+// one streaming pass over the table (k_table_sums) and K Chebyshev steps
+// from u, the last adding v as its source term: about 4K + 4 vector
+// transfers instead of the 3(M - 1) of the Clenshaw recurrence.  Returns
+// xi's buffer.
+template <class S>
+int stoch_table(fkpm_handle* h, int M, const double* c, const S** xi_out) {
+  const int K = (M + 1) / 2;
+  const KCfg& kc = h->kc[K_MOM];
+  const auto& ks = h->ks<S>(K_MOM);
+  const dim3 grid(kc.nwg), block(fkd::WG);
+  const size_t vb = h->vbytes();
+  auto cm = [&](int m) { return m < M ? c[m] : 0.0; };
+  std::vector<double>& w = h->tabc_host;
+  w.assign(2 * (size_t)(K + 1), 0.0);
+  for (int i = 0; i <= K; i++) {
+    w[i] = (i < K ? cm(i) : 0.0) - (i >= 1 ? cm(2 * K - i) : 0.0);  // d_i
+    w[K + 1 + i] = i < K ? cm(K + i) : 0.0;                         // e_i
+  }
+  CHK(h->tabc.ensure(w.size() * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(h->tabc.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice,
+                        h->stream));
+  CHK(h->Bv[0].ensure(vb));
+  S* u = (S*)h->A[0].p;
+  S* v = (S*)h->Bv[0].p;
+  const size_t n = vb / sizeof(S);
+  CHK(launch(&fkd::k_table_sums<S>, dim3(4096), block, h->stream, (const S*)h->R.p,
+             (const S*)h->atab.p, n, K, (const double*)h->tabc.as<double>(),
+             (const double*)h->tabc.as<double>() + K + 1, u, v, n));
+  // y_0 = u, y_1 = Hs u, y_{k+1} = 2 Hs y_k - y_{k-1}; the last step writes
+  // 2 y_K + v
+  fkd::RSource<S> addv = no_rsource<S>();
+  addv.rd = v;
+  addv.cr = 1.0;
+  const S* x = u;
+  const S* z = nullptr;
+  int slot = 1;
+  for (int k = 0; k < K; k++) {
+    S* y = (S*)h->A[slot].p;
+    slot = (slot + 1) % 3;
+    const bool last = (k == K - 1);
+    const double cx = (k == 0 ? 1.0 : 2.0) * (last ? 2.0 : 1.0);
+    const double cz = k == 0 ? 0.0 : (last ? -2.0 : -1.0);
+    CHK(launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(),
+               h->ci.as<const int>(), h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk, cx,
+               cz, (double*)nullptr, kc.nt, last ? addv : no_rsource<S>()));
+    z = x;
+    x = y;
+  }
+  *xi_out = x;
+  return K + 1;
+}
+
 template <class S>
 int stoch_t(fkpm_handle* h, int M, const double* c) {
   h->a_stored = false;  // the forward method's alpha_k go to the ring
@@ -866,9 +931,13 @@ int stoch_t(fkpm_handle* h, int M, const double* c) {
   const KCfg& ko = h->kc[K_OUT];
   const auto& kso = h->ks<S>(K_OUT);
   CHK(timing_begin(h));
-  if (h->stoch_method == FKPM_STOCH_CLENSHAW) {
+  const bool table = h->stoch_method == FKPM_STOCH_AUTO && h->tab_K >= (M + 1) / 2;
+  h->stoch_used = table ? FKPM_STOCH_TABLE
+                        : h->stoch_method == FKPM_STOCH_FORWARD ? FKPM_STOCH_FORWARD
+                                                                : FKPM_STOCH_CLENSHAW;
+  if (h->stoch_used != FKPM_STOCH_FORWARD) {
     const S* xi = nullptr;
-    const int launches = stoch_clenshaw<S>(h, M, c, &xi);
+    const int launches = table ? stoch_table<S>(h, M, c, &xi) : stoch_clenshaw<S>(h, M, c, &xi);
     if (launches < 0) return launches;
     CHK(timing_end(h, &h->ms_stoch, &h->n_stoch, launches));
     CHK(zero_doubles(h, h->dacc.p, dcount));
@@ -1550,6 +1619,7 @@ int fkpm_set_H(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, cons
   HIPCHK(hipStreamSynchronize(h->stream));
   h->h_set = true;
   h->alpha_M = 0;
+  h->tab_K = 0;
   if (geo_change) h->r_dirty = true;
   return 0;
 }
@@ -1568,6 +1638,7 @@ int fkpm_set_R(fkpm_handle* h, int n, int s, const void* R_colmajor) {
   h->r_set = true;
   h->r_dirty = true;
   h->alpha_M = 0;
+  h->tab_K = 0;
   return 0;
 }
 
@@ -1763,6 +1834,12 @@ int fkpm_get_alpha_table(fkpm_handle* h, int* stored) {
   return 0;
 }
 
+int fkpm_get_stoch_method(fkpm_handle* h, int* method) {
+  if (!h || !method) return fail(FKPM_E_ARG, "null argument");
+  *method = h->stoch_used;
+  return 0;
+}
+
 int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep) {
   if (!h || !sweep) return fail(FKPM_E_ARG, "null argument");
   if (kind < 0 || kind >= NKIND) return fail(FKPM_E_ARG, "bad kernel kind %d", kind);
@@ -1805,7 +1882,7 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
   if (!h) return fail(FKPM_E_ARG, "null handle");
   switch (option) {
     case FKPM_OPT_STOCH_METHOD:
-      if (value != FKPM_STOCH_CLENSHAW && value != FKPM_STOCH_FORWARD)
+      if (value != FKPM_STOCH_CLENSHAW && value != FKPM_STOCH_FORWARD && value != FKPM_STOCH_AUTO)
         return fail(FKPM_E_ARG, "bad stoch_matrix method %d", value);
       h->stoch_method = value;
       return 0;
@@ -1857,18 +1934,24 @@ int fkpm_bandwidth_probe(fkpm_handle* h, size_t bytes, int reps, double* gbs) {
   CHK(b.ensure(bytes));
   const size_t n = bytes / 16;
   CHK(zero_doubles(h, a.p, n * 2));
-  const int grid = 1024;
-  CHK(launch(&fkd::k_copy16, dim3(grid), dim3(fkd::WG), h->stream, (const double2*)a.p,
-             (double2*)b.p, n));
-  HIPCHK(hipEventRecord(h->ev0, h->stream));
-  for (int r = 0; r < reps; r++)
-    CHK(launch(&fkd::k_copy16, dim3(grid), dim3(fkd::WG), h->stream, (const double2*)a.p,
-               (double2*)b.p, n));
-  HIPCHK(hipEventRecord(h->ev1, h->stream));
-  HIPCHK(hipEventSynchronize(h->ev1));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  // the plain copy (1024 workgroups), then the streaming one (4096); the
+  // faster of the two is the ceiling
+  double best = 0.0;
+  for (int v = 0; v < 2; v++) {
+    auto fn = v == 0 ? &fkd::k_copy16 : &fkd::k_copy16_nt;
+    const int grid = v == 0 ? 1024 : 4096;
+    CHK(launch(fn, dim3(grid), dim3(fkd::WG), h->stream, (const double2*)a.p, (double2*)b.p, n));
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int r = 0; r < reps; r++)
+      CHK(launch(fn, dim3(grid), dim3(fkd::WG), h->stream, (const double2*)a.p, (double2*)b.p,
+                 n));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    best = std::max(best, 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9);
+  }
+  *gbs = best;
   return 0;
 }
 

@@ -258,9 +258,20 @@ class HipEngine:
         call("fkpm_autotune", self._h)
 
     def set_stoch_method(self, method: str) -> None:
-        """'clenshaw' (default) or 'forward' (the reference's summation order)."""
-        v = {"clenshaw": _capi.STOCH_CLENSHAW, "forward": _capi.STOCH_FORWARD}[method]
+        """'auto' (default: from the forward vectors a preceding moments kept,
+        else Clenshaw), 'clenshaw' or 'forward' (the reference's summation
+        order)."""
+        v = {"auto": _capi.STOCH_AUTO, "clenshaw": _capi.STOCH_CLENSHAW,
+             "forward": _capi.STOCH_FORWARD}[method]
         call("fkpm_set_option", self._h, _capi.OPT_STOCH_METHOD, v)
+
+    def stoch_method(self) -> str:
+        """The method the last stoch_matrix ran: 'clenshaw', 'forward',
+        'table' or '' (none yet)."""
+        v = C.c_int()
+        call("fkpm_get_stoch_method", self._h, C.byref(v))
+        return {_capi.STOCH_CLENSHAW: "clenshaw", _capi.STOCH_FORWARD: "forward",
+                _capi.STOCH_TABLE: "table"}.get(v.value, "")
 
     def set_autotune_on_use(self, on: bool) -> None:
         call("fkpm_set_option", self._h, _capi.OPT_AUTOTUNE, 1 if on else 0)

@@ -153,7 +153,8 @@ int fkpm_get_config(fkpm_handle* h, int* lanes, int* kq, int* s_padded, int* n_w
 int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 
 /* Options (fkpm_set_option). */
-#define FKPM_OPT_STOCH_METHOD 1 /* value: FKPM_STOCH_CLENSHAW (default) or FKPM_STOCH_FORWARD */
+#define FKPM_OPT_STOCH_METHOD 1 /* value: FKPM_STOCH_AUTO (default), FKPM_STOCH_CLENSHAW or
+                                   FKPM_STOCH_FORWARD */
 #define FKPM_OPT_AUTOTUNE 2     /* value 1: tune the kernel geometry at first use of a shape */
 #define FKPM_OPT_MFMA 3         /* value 1 (default): matrix-core (MFMA) kernels where the block
                                    shape has them (complex 8x8 blocks); 0: vector-unit kernels */
@@ -172,6 +173,12 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 #define FKPM_STOCH_CLENSHAW 0   /* xi by the Clenshaw recurrence: one 2-vector pass per order */
 #define FKPM_STOCH_FORWARD 1    /* xi accumulated in FP64 along the forward recursion, the
                                    reference's summation order (engine_cpu.cpp:236-244) */
+#define FKPM_STOCH_TABLE 2      /* (reported only) xi from the forward vectors a preceding
+                                   fkpm_moments kept (FKPM_OPT_STORE_ALPHA), by
+                                   T_{K+j} = 2 T_K T_j - T_{K-j}: one pass over the table and
+                                   K = ceil(M/2) Chebyshev steps instead of M - 1 */
+#define FKPM_STOCH_AUTO 3       /* FKPM_STOCH_TABLE when the held forward vectors (same H and
+                                   R) reach alpha_K, else FKPM_STOCH_CLENSHAW */
 int fkpm_set_option(fkpm_handle* h, int option, int value);
 
 /* Kernel kinds of fkpm_get_kernel_config. */
@@ -209,16 +216,21 @@ int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep);
  * next fkpm_autodiff_matrix (FKPM_OPT_STORE_ALPHA). */
 int fkpm_get_alpha_table(fkpm_handle* h, int* stored);
 
+/* *method = the FKPM_STOCH_* method the last fkpm_stoch_matrix ran (-1 if
+ * none yet). */
+int fkpm_get_stoch_method(fkpm_handle* h, int* method);
+
 /* r_compact: R has at most one nonzero per row (probing colours) and the
  * Clenshaw source term reads the compact form; tuned: the geometry came from
  * the autotuner; moments_M: M of the held moments state (0 if none). */
 int fkpm_get_state(fkpm_handle* h, int* r_compact, int* tuned, int* moments_M);
 
 /* Bandwidth probe: copies `bytes` (a multiple of 16) between two device
- * buffers with a plain streaming kernel (k_copy16), `reps` times, and reports
- * the achieved read+write GB/s.  The measured ceiling the step kernels are set
- * against, and the known-size kernel that calibrates HBM byte counters.  No
- * counterpart in the reference. */
+ * buffers `reps` times with a plain copy kernel (k_copy16) and with a
+ * streaming one (k_copy16_nt: four packets per thread, nontemporal hints),
+ * and reports the better read+write GB/s.  The measured ceiling the step
+ * kernels are set against; k_copy16 is also the known-size kernel that
+ * calibrates HBM byte counters.  No counterpart in the reference. */
 int fkpm_bandwidth_probe(fkpm_handle* h, size_t bytes, int reps, double* gbs);
 
 /* ------------------------------------------------------------------------

@@ -241,6 +241,51 @@ def test_autodiff_alpha_table_and_recompute(oracle, store, name, shape, b, J, W,
         assert rel_err(DD.val, orc.autodiff_matrix(c, DD)) < FP64_TOL, (name, store)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("name,shape,b,J,W,ob,periods,real", [CASES[0], CASES[2], CASES[4]],
+                         ids=[CASES[k][0] for k in (0, 2, 4)])
+def test_stoch_matrix_from_table(oracle, prec, name, shape, b, J, W, ob, periods, real):
+    """stoch_matrix from the forward vectors a preceding moments(M) kept
+    (xi = v + 2 T_K(Hs) u), for M' <= M coefficients, odd and tiny M'; M' > M,
+    a new H and an explicit Clenshaw request fall back to the recurrence."""
+    from fastkpm_amd import EnergyScale
+    H, R = _case(oracle, shape, b, seed=23, J=J, W=W, orbital_blocks=ob, periods=periods, real=real)
+    lam = np.linalg.eigvalsh(H.to_dense())
+    lo, hi = lam[0] - 0.1, lam[-1] + 0.1
+    M = 48
+    dtype = "double" if real else "cx_double"
+    eng, orc, D = _run_pair(oracle, H, R, lo, hi, M, dtype=dtype, prec=prec)
+    eng.set_store_alpha(2)
+    tol = FP64_TOL if prec == "fp64" else 1e-4
+    c = fermi_coeffs(oracle, 2 * M, lo, hi, kT=0.05, kind=1)
+    eng.moments(M)
+    assert eng.alpha_table()
+    for m in (M, M - 1, 30, 3, 2):
+        eng.stoch_matrix(c[:m], D)
+        assert eng.stoch_method() == "table", m
+        assert rel_err(D.val, orc.stoch_matrix(c[:m], D)) < tol, (name, m)
+    # autodiff_matrix still needs its moments(M); the table serves both
+    eng.moments(M)
+    orc.moments(M)
+    cE = fermi_coeffs(oracle, M, lo, hi, kT=0.05)
+    eng.autodiff_matrix(cE, D)
+    assert rel_err(D.val, orc.autodiff_matrix(cE, D)) < tol
+    eng.stoch_matrix(c[:M], D)
+    assert eng.stoch_method() == "table"
+    eng.stoch_matrix(c[:M + 6], D)  # beyond alpha_{M/2}
+    assert eng.stoch_method() == "clenshaw"
+    assert rel_err(D.val, orc.stoch_matrix(c[:M + 6], D)) < tol
+    eng.set_stoch_method("clenshaw")
+    eng.stoch_matrix(c[:M], D)
+    assert eng.stoch_method() == "clenshaw"
+    eng.set_stoch_method("auto")
+    eng.set_H(H, EnergyScale(lo - 0.05, hi))  # another Hs: the table is stale
+    orc.set_H(H, lo - 0.05, hi)
+    eng.stoch_matrix(c[:M], D)
+    assert eng.stoch_method() == "clenshaw"
+    assert rel_err(D.val, orc.stoch_matrix(c[:M], D)) < tol
+
+
 def test_option_refusals():
     from fastkpm_amd._capi import FkpmError, OPT_SWEEP, OPT_MFMA, call
     eng = _engine()
