@@ -332,7 +332,8 @@ class HipEngine:
 
 
 def bandwidth_probe(eng: "HipEngine", nbytes: int = 1 << 30, reps: int = 10) -> float:
-    """Read+write GB/s of a plain streaming copy of nbytes on eng's device."""
+    """Read+write GB/s of the faster of two copy kernels (plain, and four
+    packets per thread with nontemporal hints) over nbytes on eng's device."""
     g = C.c_double()
     call("fkpm_bandwidth_probe", eng.handle, int(nbytes), int(reps), C.byref(g))
     return g.value

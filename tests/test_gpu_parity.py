@@ -452,3 +452,15 @@ def test_autotuned_engine(oracle, prec, store):
     assert eng2.state()["tuned"]
     for k in range(6):
         assert eng2.kernel_config(k) == eng.kernel_config(k)
+
+
+def test_bandwidth_probe():
+    """The copy-bandwidth probe: a finite rate for a ragged size, refusals."""
+    from fastkpm_amd import bandwidth_probe
+    from fastkpm_amd._capi import FkpmError
+    eng = _engine()
+    g = bandwidth_probe(eng, (64 << 20) + 16 * 1000, 3)  # not a multiple of a workgroup's span
+    assert np.isfinite(g) and g > 50.0, g
+    for nbytes, reps in ((1000 * 16 + 8, 1), (0, 1), (1 << 20, 0)):
+        with pytest.raises(FkpmError):
+            bandwidth_probe(eng, nbytes, reps)
