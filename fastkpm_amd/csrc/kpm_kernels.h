@@ -143,7 +143,7 @@ __device__ __forceinline__ double wave_sum(double v) { return group_sum<WAVE>(v)
 
 // The value held by lane l ^ M (M a power of two < 64): quad_perm DPP for
 // M = 1, 2; ds_swizzle in bitmask mode (lane ((l & 0x1F) ^ M) of the 32-lane
-// half) for M = 4, 8, 16; a shuffle for M = 32.
+// half) for M = 4, 8, 16; a shuffle for M = 32.  Doubles move as two dwords.
 template <int M>
 __device__ __forceinline__ double xor_move(double v) {
   if constexpr (M == 1) {
@@ -161,21 +161,36 @@ __device__ __forceinline__ double xor_move(double v) {
   }
 }
 
+template <int M>
+__device__ __forceinline__ float xor_move(float v) {
+  const int b = __builtin_bit_cast(int, v);
+  if constexpr (M == 1) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, false));
+  } else if constexpr (M == 2) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, false));
+  } else if constexpr (M < 32) {
+    constexpr int PAT = (M << 10) | 0x1F;
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(b, PAT));
+  } else {
+    return __shfl_xor(v, M, WAVE);
+  }
+}
+
 // Reduce-scatter of CNT values over the lanes M, 2M, ... < L of a group: at
 // each level a lane keeps the half of its values selected by lane bit M, adds
 // the partner's copy of that half and sends the other half; once one value
 // is left the remaining levels are plain butterflies.  Afterwards lane l holds
 // in v[0 .. max(NV/L, 1)) the group sums of indices rs_base<L, NV>(l) + t.
-template <int M, int L, int CNT>
-__device__ __forceinline__ void rs_level(double* v, int l) {
+template <int M, int L, int CNT, typename T>
+__device__ __forceinline__ void rs_level(T* v, int l) {
   if constexpr (M < L) {
     if constexpr (CNT > 1) {
       constexpr int H = CNT / 2;
       const bool up = (l & M) != 0;
 #pragma unroll
       for (int t = 0; t < H; t++) {
-        const double send = up ? v[t] : v[t + H];
-        const double keep = up ? v[t + H] : v[t];
+        const T send = up ? v[t] : v[t + H];
+        const T keep = up ? v[t + H] : v[t];
         v[t] = keep + xor_move<M>(send);
       }
       rs_level<2 * M, L, H>(v, l);
@@ -186,8 +201,8 @@ __device__ __forceinline__ void rs_level(double* v, int l) {
   }
 }
 
-template <int L, int NV>
-__device__ __forceinline__ void reduce_scatter(double* v, int l) {
+template <int L, int NV, typename T>
+__device__ __forceinline__ void reduce_scatter(T* v, int l) {
   static_assert((NV & (NV - 1)) == 0, "power of two");
   rs_level<1, L, NV>(v, l);
 }
@@ -841,7 +856,8 @@ __device__ __forceinline__ void step_autodiff_dev(
             if constexpr (FUSED_D && NV <= 16) {
               // D_p(bi, bj) += fD * sum_s alpha_m[B i + bi, s] conj(beta_{m+1}[B j + bj, s]):
               // per-lane partial sums of the whole block, then one reduce-scatter
-              double v[NV];
+              // over the group in the storage precision, added to D in FP64
+              A v[NV];
 #pragma unroll
               for (int bj = 0; bj < B; bj++)
 #pragma unroll
@@ -861,8 +877,8 @@ __device__ __forceinline__ void step_autodiff_dev(
                       }
                     }
                   const int de = bj * B + bi;  // column-major element index in the block
-                  v[de * HS] = (double)re;
-                  if constexpr (CX) v[de * HS + 1] = (double)im;
+                  v[de * HS] = re;
+                  if constexpr (CX) v[de * HS + 1] = im;
                 }
               reduce_scatter<L, NV>(v, l);
               constexpr int CNT = NV >= L ? NV / L : 1;
@@ -870,7 +886,7 @@ __device__ __forceinline__ void step_autodiff_dev(
                 const int base = rs_base<L, NV>(l);
                 double* d = dacc + (size_t)p * NV + base;
 #pragma unroll
-                for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], dpre_get(dp, t, d + t));
+                for (int t = 0; t < CNT; t++) d[t] = fma(fD, (double)v[t], dpre_get(dp, t, d + t));
               }
             } else if constexpr (FUSED_D) {
               double own[GE::DOWN][HS];
@@ -1012,7 +1028,7 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
           (void)p;
           (void)xv;
         } else if constexpr (NV <= 16) {
-          double v[NV];
+          A v[NV];  // reduced in the storage precision, added to D in FP64
 #pragma unroll
           for (int bj = 0; bj < B; bj++)
 #pragma unroll
@@ -1032,8 +1048,8 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
                   }
                 }
               const int de = bj * B + bi;  // column-major element index in the block
-              v[de * HS] = (double)re;
-              if constexpr (CX) v[de * HS + 1] = (double)im;
+              v[de * HS] = re;
+              if constexpr (CX) v[de * HS + 1] = im;
             }
           reduce_scatter<L, NV>(v, l);
           constexpr int CNT = NV >= L ? NV / L : 1;
@@ -1041,7 +1057,7 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
             const int base = rs_base<L, NV>(l);
             double* d = dacc + (size_t)p * NV + base;
 #pragma unroll
-            for (int t = 0; t < CNT; t++) d[t] = fma(fD, v[t], dpre_get(dp, t, d + t));
+            for (int t = 0; t < CNT; t++) d[t] = fma(fD, (double)v[t], dpre_get(dp, t, d + t));
           }
         } else {
           double own[GE::DOWN][HS];
