@@ -558,6 +558,8 @@ int timing_end(fkpm_handle* h, double* ms, int* cnt, int launches) {
 // ---------------------------------------------------------------------------
 template <class S>
 int tune_t(fkpm_handle* h) {
+  h->tab_K = 0;  // the table may serve as tuning slots below
+  h->alpha_M = 0;
   const int HS = h->cx ? 2 : 1;
   const size_t vb = h->vbytes();
   CHK(h->Bv[0].ensure(vb));
