@@ -371,6 +371,10 @@ KCfg default_cfg(fkpm_handle* h, int kind) {
   }
   c.maxwg = 4096;
   c.nt = 0;
+  // one front in segments of two passes: what the tuner picks on the 2-D and
+  // 3-D lattices measured (C2 FP64: forward step 91 -> 81 us, autodiff 188
+  // -> 166 us against per-XCD regions)
+  c.sweep = 2;
   if (mfma_avail(h, kind)) {
     c.mfma = 1;
     // row-resident: the forward step always, the autodiff step in FP64
@@ -1570,6 +1574,9 @@ int fkpm_dev_open(int device, int dtype, int prec, fkpm_handle** out) {
   h->prec = prec;
   h->cx = dtype_complex(dtype);
   h->single = (prec == FKPM_PREC_FP32);
+  // FKPM_AUTOTUNE=1: tune at the first use of a shape (FKPM_OPT_AUTOTUNE for
+  // callers that cannot reach fkpm_set_option, such as the C++ API)
+  if (const char* e = getenv("FKPM_AUTOTUNE")) h->autotune_on_use = e[0] != '\0' && e[0] != '0';
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
     delete h;

@@ -155,7 +155,9 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 /* Options (fkpm_set_option). */
 #define FKPM_OPT_STOCH_METHOD 1 /* value: FKPM_STOCH_AUTO (default), FKPM_STOCH_CLENSHAW or
                                    FKPM_STOCH_FORWARD */
-#define FKPM_OPT_AUTOTUNE 2     /* value 1: tune the kernel geometry at first use of a shape */
+#define FKPM_OPT_AUTOTUNE 2     /* value 1: tune the kernel geometry at first use of a shape
+                                   (default 0; the environment variable FKPM_AUTOTUNE=1 at
+                                   fkpm_dev_open sets 1) */
 #define FKPM_OPT_MFMA 3         /* value 1 (default): matrix-core (MFMA) kernels where the block
                                    shape has them (complex 8x8 blocks); 0: vector-unit kernels */
 #define FKPM_OPT_STORE_ALPHA 5  /* value 2: fkpm_moments keeps every alpha_k when the table
@@ -169,7 +171,7 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 #define FKPM_OPT_SWEEP 4        /* value 0: every kernel kind sweeps the rows per XCD; s in
                                    1..64: as one front in segments of s passes per XCD
                                    (fkpm_get_kernel_sweep); -1 (default): the configuration's
-                                   own (the tuner's choice, else per XCD) */
+                                   own (the tuner's choice, else one front with s = 2) */
 #define FKPM_STOCH_CLENSHAW 0   /* xi by the Clenshaw recurrence: one 2-vector pass per order */
 #define FKPM_STOCH_FORWARD 1    /* xi accumulated in FP64 along the forward recursion, the
                                    reference's summation order (engine_cpu.cpp:236-244) */

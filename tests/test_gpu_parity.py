@@ -464,3 +464,25 @@ def test_bandwidth_probe():
     for nbytes, reps in ((1000 * 16 + 8, 1), (0, 1), (1 << 20, 0)):
         with pytest.raises(FkpmError):
             bandwidth_probe(eng, nbytes, reps)
+
+
+@pytest.mark.parametrize("env", [None, "1"])
+def test_autotune_from_environment(oracle, monkeypatch, env):
+    """FKPM_AUTOTUNE=1 at open tunes a shape at its first use (the switch for
+    callers without an option call); without it the default configuration
+    runs.  Shapes of their own, so that no earlier test tuned them."""
+    if env is None:
+        monkeypatch.delenv("FKPM_AUTOTUNE", raising=False)
+        shape = (10, 6)
+    else:
+        monkeypatch.setenv("FKPM_AUTOTUNE", env)
+        shape = (12, 6)
+    H, R, es, orc = _pair(oracle, shape=shape, J=1.0)
+    eng = _engine()
+    eng.set_R(R)
+    eng.set_H(H, es)
+    M = 40
+    mu = eng.moments(M)
+    assert eng.state()["tuned"] is (env is not None)
+    assert rel_err(mu, orc.moments(M)) < FP64_TOL
+    assert eng.kernel_config(0)["sweep"] in ((2,) if env is None else (0, 1, 2, 8))
