@@ -1,12 +1,15 @@
-"""Properties at the full size of the benchmark's lattice workload (256 x 256
-sites x 2 spin orbitals, N = 131072, 64 probing colours), where the oracle
-would take minutes: relations the mathematics fixes whatever the size, on a
-short expansion (M = 64) so the test runs in seconds.
+"""Properties at the full sizes of the benchmark's workloads, where the oracle
+would take minutes: C2 (256 x 256 sites x 2 spin orbitals, N = 131072, 64
+probing colours), C3 (64^3 x 2, N = 524288, 128 colours) and C4 (32^3 sites
+x 8 orbitals, N = 262144, 256 colours, matrix cores): relations the
+mathematics fixes whatever the size, on a short expansion (M = 64) so each
+test runs in seconds.
 
 - probing R: mu_0 = sum |R|^2 = N;
 - stoch_matrix: the diagonal of D = (R xi^+ + xi R^+) / 2 sums to
-  Re <R, xi> = sum_m c_m mu_m, and the result from the kept forward vectors
-  equals the Clenshaw recurrence's;
+  Re <R, xi> = sum_m c_m mu_m, D is Hermitian, and the result from the kept
+  forward vectors equals the Clenshaw recurrence's (C4, with no table: the
+  Clenshaw result equals the forward order's);
 - autodiff_matrix: sum_r D_rr = dE/d(eps) for H + eps I (central finite
   difference of E = sum_m c_m mu_m at fixed energy scale);
 - FP32-mixed energy within 1e-6 of FP64.
@@ -23,11 +26,14 @@ pytestmark = pytest.mark.gpu
 M = 64
 
 
-@pytest.fixture(scope="module")
-def workload():
-    from fastkpm_amd import lattice
-    H, groups, cfg = lattice.make_config("C2", seed=2024)
-    return H, groups, cfg
+@pytest.fixture(scope="module", params=["C2", "C3", "C4"])
+def workload(request):
+    from fastkpm_amd import HipEngine, lattice
+    H, groups, cfg = lattice.make_config(request.param, seed=2024)
+    eng = HipEngine("cx_double", "fp64")
+    es = eng.energy_scale(H, 0.05)  # Lanczos bounds with 5 % slack
+    eng.close()
+    return H, groups, cfg, es
 
 
 def _diag_sum(D) -> float:
@@ -48,19 +54,18 @@ def _engine(H, groups, es, prec="fp64", store=1):
 
 
 def test_fullsize_moments_and_stoch_trace(workload):
-    from fastkpm_amd import EnergyScale, host
-    H, groups, cfg = workload
-    es = EnergyScale(-6.5, 6.5)
+    from fastkpm_amd import host
+    H, groups, cfg, es = workload
     N = H.n_rows * H.b_len
     eng = _engine(H, groups, es, store=2)
     mu = eng.moments(M)
-    assert eng.alpha_table()
+    table = eng.alpha_table()  # not with the matrix-core autodiff step (C4)
     assert abs(mu[0] - N) <= 1e-12 * N, mu[0]
     c = host.expansion_coefficients(M, 2 * M, host.FERMI_DENSITY, cfg["kT"], cfg["mu"], es.lo,
                                     es.hi)
     D = H.like()
     eng.stoch_matrix(c, D)
-    assert eng.stoch_method() == "table"
+    assert eng.stoch_method() == ("table" if table else "clenshaw")
     tr = _diag_sum(D)
     expect = float(np.dot(c, mu))
     assert abs(tr - expect) <= 1e-10 * abs(expect), (tr, expect)
@@ -68,19 +73,17 @@ def test_fullsize_moments_and_stoch_trace(workload):
     k = int(np.nonzero(D.row_idx != D.col_idx)[0][0])
     kt = D.find_index(int(D.col_idx[k]), int(D.row_idx[k]))
     assert np.allclose(D.block(k), D.block(kt).conj().T, rtol=0, atol=1e-14 * np.abs(D.val).max())
-    # the same D from the Clenshaw recurrence
-    eng.set_stoch_method("clenshaw")
+    # the same D from the other method (Clenshaw, or the forward order)
+    eng.set_stoch_method("clenshaw" if table else "forward")
     D2 = H.like()
     eng.stoch_matrix(c, D2)
-    assert eng.stoch_method() == "clenshaw"
     assert rel_err(D2.val, D.val) < 1e-11
     eng.close()
 
 
 def test_fullsize_autodiff_finite_difference(workload):
-    from fastkpm_amd import EnergyScale, host
-    H, groups, cfg = workload
-    es = EnergyScale(-6.5, 6.5)
+    from fastkpm_amd import host
+    H, groups, cfg, es = workload
     c = host.expansion_coefficients(M, 2 * M, host.FERMI_ENERGY, cfg["kT"], cfg["mu"], es.lo,
                                     es.hi)
     eng = _engine(H, groups, es)
@@ -106,9 +109,8 @@ def test_fullsize_autodiff_finite_difference(workload):
 
 
 def test_fullsize_fp32_energy(workload):
-    from fastkpm_amd import EnergyScale, host
-    H, groups, cfg = workload
-    es = EnergyScale(-6.5, 6.5)
+    from fastkpm_amd import host
+    H, groups, cfg, es = workload
     c = host.expansion_coefficients(M, 2 * M, host.FERMI_ENERGY, cfg["kT"], cfg["mu"], es.lo,
                                     es.hi)
     e = []
