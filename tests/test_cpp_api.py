@@ -27,9 +27,14 @@ def test_cpp_api_host_checks():
 
 
 @pytest.mark.gpu
-def test_cpp_api_device_flow_matches_oracle(oracle, tmp_path):
+@pytest.mark.parametrize("autotune", ["0", "1"])
+def test_cpp_api_device_flow_matches_oracle(oracle, tmp_path, autotune):
+    """autotune 1: FKPM_AUTOTUNE=1, the tuner runs at the first use of each
+    shape inside the C++ engine."""
+    import os
+    env = dict(os.environ, FKPM_AUTOTUNE=autotune)
     r = subprocess.run([_bin(), "device", str(tmp_path)], capture_output=True, text=True,
-                       timeout=120)
+                       timeout=120, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     from fastkpm_amd import SpMatBsr
     meta = (tmp_path / "meta.txt").read_text().split()
