@@ -272,13 +272,18 @@ struct Geo {
   static constexpr int DE = B * B;               // elements per D block
   static constexpr int DOWN = (DE + L - 1) / L;  // D elements owned per lane
   // Neighbour blocks gathered per batch for NX gathered vectors: a batch holds
-  // U * (NX*B*KQ*NS + B*B*HS) values; about 48 keep the wave's registers in
-  // check while giving several blocks of loads in flight.
-  template <int NX>
+  // U * (NX*B*KQ*NS + B*B*HS) values; a budget of about 48 (up to 4 blocks)
+  // keeps the registers of the autodiff steps in check while giving several
+  // blocks of loads in flight; the forward steps, with fewer live values,
+  // take 64 (up to 8 blocks: a 2-D lattice row in one batch on 64 lanes,
+  // FP64 forward step 80 -> 79 us at C2; the autodiff step lost a wave per
+  // SIMD and 4 us with it).
+  template <int NX, int BUDGET = 48>
   static constexpr int batch() {
     constexpr int per = NX * B * KQ * NS + B * B * HS;
-    constexpr int u = 48 / per;
-    return u < 1 ? 1 : (u > 4 ? 4 : u);
+    constexpr int u = BUDGET / per;
+    constexpr int cap = BUDGET > 48 ? 8 : 4;
+    return u < 1 ? 1 : (u > cap ? cap : u);
   }
 };
 
@@ -366,7 +371,7 @@ __device__ __forceinline__ int first_row(int i) {
 struct NoPre {};
 
 template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, bool OWN, class F,
-          class P>
+          class P, int BUDGET = 48>
 __device__ __forceinline__ void gather_row_pre(const int* __restrict__ ci,
                                                const S* __restrict__ hv, const S* const (&X)[NX],
                                                size_t rs, size_t qo, int p0, int p1,
@@ -375,7 +380,7 @@ __device__ __forceinline__ void gather_row_pre(const int* __restrict__ ci,
                                                const A (&own)[KQ][B][Vec16<S>::N]) {
   using GE = Geo<S, CX, B, L, KQ>;
   constexpr int NS = GE::NS, HS = GE::HS, BBS = GE::BBS;
-  constexpr int U = GE::template batch<NX>();
+  constexpr int U = GE::template batch<NX, BUDGET>();
   using PV = decltype(pre(0));
   int jn[U];
 #pragma unroll
@@ -447,23 +452,16 @@ __device__ __forceinline__ void gather_row_pre(const int* __restrict__ ci,
   }
 }
 
-template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, bool OWN, class F>
-__device__ __forceinline__ void gather_row_impl(const int* __restrict__ ci,
-                                                const S* __restrict__ hv, const S* const (&X)[NX],
-                                                size_t rs, size_t qo, int p0, int p1,
-                                                A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk,
-                                                int iown, const A (&own)[KQ][B][Vec16<S>::N]) {
-  gather_row_pre<S, A, CX, B, L, KQ, NX, OWN>(
-      ci, hv, X, rs, qo, p0, p1, acc, [&](int p, int jb, auto& xv, NoPre) { blk(p, jb, xv); },
-      [](int) { return NoPre{}; }, iown, own);
-}
-
+// The forward steps' gather (no prefetched values, the larger batch budget).
 template <typename S, typename A, bool CX, int B, int L, int KQ, int NX, class F>
 __device__ __forceinline__ void gather_row(const int* __restrict__ ci, const S* __restrict__ hv,
                                            const S* const (&X)[NX], size_t rs, size_t qo, int p0,
                                            int p1, A (&acc)[NX][KQ][B][Vec16<S>::N], F&& blk) {
   const A none[KQ][B][Vec16<S>::N] = {};
-  gather_row_impl<S, A, CX, B, L, KQ, NX, false>(ci, hv, X, rs, qo, p0, p1, acc, blk, -1, none);
+  auto nopre = [](int) { return NoPre{}; };
+  auto blk4 = [&](int p, int jb, auto& xv, NoPre) { blk(p, jb, xv); };
+  gather_row_pre<S, A, CX, B, L, KQ, NX, false, decltype(blk4)&, decltype(nopre)&, 64>(
+      ci, hv, X, rs, qo, p0, p1, acc, blk4, nopre, -1, none);
 }
 
 // Own rows of block row i: v[k][bi] = X[B i + bi] packets of this lane.
