@@ -234,7 +234,7 @@ struct fkpm_handle {
     DevBuf* bufs[] = {&rp,    &ci,    &hv,      &rowblk, &hraw, &rraw,  &rcol,     &rval,
                       &rflag, &R,     &A[0],    &A[1],   &A[2], &Bv[0], &Bv[1],    &Bv[2],
                       &xi,    &partial, &psum,  &tune_dacc, &drp, &dci,  &tpos,     &dacc,
-                      &dout,  &atab};
+                      &dout,  &atab,  &tabc};
     for (DevBuf* d : bufs) d->release();
     DevBuf* m2bufs[] = {&j_rp[0], &j_ci[0], &j_rb[0], &j_raw[0], &j_val[0], &j_rp[1], &j_ci[1],
                         &j_rb[1], &j_raw[1], &j_val[1], &m2_a, &m2_b, &m2_ring, &m2_c, &m2_part};
