@@ -353,14 +353,25 @@ __device__ __forceinline__ void mfma_step_rr_dev(
     const int p0 = rp[i], deg = rp[i + 1] - p0;
     S a[DM][4];
     uint32_t o0[DM];
+    if (deg > 0) {
+      // every block's index and H values in one batch of loads (padding
+      // slots: the last block, weight 0)
+      int jj[DM];
+      HRaw<S> hr[DM];
 #pragma unroll
-    for (int u = 0; u < DM; u++) {
-      if (deg > 0) {
-        const int pu = p0 + min(u, deg - 1);  // padding slots: the last block, weight 0
-        const int j = ci[pu];
-        mfma_a_of<S>(mfma_hraw<S>(hv + (size_t)pu * 128, m), m, u < deg ? S(1) : S(0), a[u]);
-        o0[u] = ((uint32_t)(8 * j + m.q) * rs + 2u * m.c) * ES;
-      } else {
+      for (int u = 0; u < DM; u++) {
+        const int pu = p0 + min(u, deg - 1);
+        jj[u] = ci[pu];
+        hr[u] = mfma_hraw<S>(hv + (size_t)pu * 128, m);
+      }
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        mfma_a_of<S>(hr[u], m, u < deg ? S(1) : S(0), a[u]);
+        o0[u] = ((uint32_t)(8 * jj[u] + m.q) * rs + 2u * m.c) * ES;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
 #pragma unroll
         for (int k = 0; k < 4; k++) a[u][k] = S(0);
         o0[u] = 0u;
@@ -496,15 +507,25 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
     const int p0 = rp[i], deg = rp[i + 1] - p0;
     S a[DM][4];
     uint32_t o0[DM], od[DM];
+    if (deg > 0) {
+      // every block's index and H values in one batch of loads
+      int jj[DM];
+      HRaw<S> hr[DM];
 #pragma unroll
-    for (int u = 0; u < DM; u++) {
-      if (deg > 0) {
+      for (int u = 0; u < DM; u++) {
         const int pu = p0 + min(u, deg - 1);
-        const int j = ci[pu];
-        mfma_a_of<S>(mfma_hraw<S>(hv + (size_t)pu * 128, m), m, u < deg ? S(1) : S(0), a[u]);
-        o0[u] = ((uint32_t)(8 * j + m.q) * rs + 2u * m.c) * ES;
-        od[u] = ((uint32_t)(8 * j + dorb) * rs + (uint32_t)dpart + 2u * m.q) * ES;
-      } else {
+        jj[u] = ci[pu];
+        hr[u] = mfma_hraw<S>(hv + (size_t)pu * 128, m);
+      }
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        mfma_a_of<S>(hr[u], m, u < deg ? S(1) : S(0), a[u]);
+        o0[u] = ((uint32_t)(8 * jj[u] + m.q) * rs + 2u * m.c) * ES;
+        od[u] = ((uint32_t)(8 * jj[u] + dorb) * rs + (uint32_t)dpart + 2u * m.q) * ES;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
 #pragma unroll
         for (int k = 0; k < 4; k++) a[u][k] = S(0);
         o0[u] = od[u] = 0u;
