@@ -1,25 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X stochastic Chebyshev engine.
 
-One step = one full KPM solve of the workload with inputs resident in HBM:
+One step = one full KPM solve of the workload with H, R and the coefficients
+resident in HBM:
   moments(M) -> E = sum c_m mu_m,
-  autodiff_matrix(c_E, D) (dE/dH on H's pattern, left on the device),
-  stoch_matrix(c_f, D')   (D_ij = f(H)_ij on H's pattern, left on the device).
+  autodiff_matrix(c_E, D) (dE/dH on H's pattern),
+  stoch_matrix(c_f, D')   (D_ij = f(H)_ij on H's pattern),
+each D copied to the host as the drop-in Engine calls return it.
 value = N*M*S / (seconds per step): Chebyshev vector updates per second.
 
-Default workload: Kondo lattice 256x256 sites x 2 spin orbitals (N = 131072,
-complex 2x2 BSR blocks, hopping -I2, on-site -J s_i.sigma with J = 2 and
-seeded random unit spins), M = 1000, S = 64 probing colours (4x8 spatial
-colouring x 2 orbitals), Fermi function at mu = 0, kT = 0.01, FP64.
+Default workload (the largest one-GPU configuration, and the one the
+FP32-mixed vs FP64 comparison is stated on): cubic lattice 64^3 sites x 2 spin
+orbitals (N = 524288, complex 2x2 BSR blocks), hopping -I2, Anderson on-site
+disorder U[-2, 2] and Kondo coupling -J s_i.sigma with J = 1 and seeded
+random unit spins, M = 2000, S = 128 probing colours (4x4x4 spatial colouring
+x 2 orbitals), Fermi function at mu = 0, kT = 0.01; FP64, with the FP32-mixed
+run of the same workload beside it.
 
-Prints one JSON line.  The CPU baseline times the FP64 oracle (single thread)
-on the same workload with M' moments (a bounded sample) and reports N*M'*S/s.
-The "serving" entry runs a seeded Poisson stream of mixed requests (energy +
-dE/dH each) through the batched engine, FP64 and FP32-mixed, and reports
-p50/p99 latency and aggregate updates/s; --serve-only runs that stream alone,
-at the full request ranges, and prints it as its own JSON line.  The
-"conductivity" entry times the two-current moments moments2_v1(M=128) of the
-x current on the same workload (FP64 and FP32-mixed) and the static sigma_xx.
+Prints one JSON line.  Besides the headline it carries: "fp32_mixed" (the same
+steps in FP32 storage with FP64 accumulation, errors against FP64); "serving"
+(a seeded Poisson stream of mixed energy + dE/dH requests over the full
+serving ranges through the batched engine, FP64 and FP32-mixed: p50/p99
+latency and aggregate updates/s; the p99 is also in "latency_ms");
+"C2" (the 2-D Kondo lattice, FP64 and FP32-mixed); "C1" (the 32x32 lattice
+solve time, which per-step launches set); "conductivity" (two-current moments
+moments2_v1(M=128) of the x current on C2 and the static sigma_xx);
+"exact_check" (KPM against exact diagonalisation on a small lattice of the
+same kind); "cpu_baseline" (the FP64 oracle on the same H and R with M'
+moments, a bounded sample, one thread and on column shards).
+--serve-only runs the serving stream alone and prints it as its own line.
 """
 from __future__ import annotations
 
@@ -110,7 +119,7 @@ def measured_traffic(config: str, prec: str, kernel: str):
     return None
 
 
-def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
+def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int, tune: bool = True):
     M = cfg["M"]
     eng = HipEngine("cx_double", prec)
     t_es = time.perf_counter()
@@ -119,7 +128,8 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     eng.set_R_correlated(groups, host.RNG(seed))
     eng.set_H(H, es)
     t_tune = time.perf_counter()
-    eng.autotune()  # setup, like a library's heuristics pass: not in the timed region
+    if tune:
+        eng.autotune()  # setup, like a library's heuristics pass: not in the timed region
     t_tune = time.perf_counter() - t_tune
     S = eng.R.shape[1]
     N = H.n_rows * H.b_len
@@ -130,9 +140,10 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     Ds = H.like()
 
     def step():
+        # as Engine::autodiff_matrix / stoch_matrix: each D ends in host memory
         mu = eng.moments(M)
-        eng.autodiff_matrix(cE, Dg, keep_on_device=True)
-        eng.stoch_matrix(cF, Ds, keep_on_device=True)
+        eng.autodiff_matrix(cE, Dg)
+        eng.stoch_matrix(cF, Ds)
         return mu
 
     for _ in range(warmup):
@@ -146,9 +157,9 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
         t0 = time.perf_counter()
         mu = eng.moments(M)
         tm = eng.timing()
-        eng.autodiff_matrix(cE, Dg, keep_on_device=True)
+        eng.autodiff_matrix(cE, Dg)
         ta = eng.timing()
-        eng.stoch_matrix(cF, Ds, keep_on_device=True)
+        eng.stoch_matrix(cF, Ds)
         ts = eng.timing()
         times.append(time.perf_counter() - t0)
         tot["moments"][0] += tm["moments_ms"]
@@ -162,18 +173,23 @@ def run_gpu(H, groups, cfg, prec: str, steps: int, warmup: int, seed: int):
     eng.set_timing(False)
     stoch_method = eng.stoch_method()  # from the kept forward vectors, or Clenshaw
     energy = float(np.dot(cE, mu))
-    eng.get_D(Ds)  # the device holds the last computed D: stoch_matrix's
-    # copy of the gradient D: one more moments + autodiff_matrix to the host
+    # the device-to-host copy of one D (pageable host memory, as a caller's
+    # SpMatBsr holds it): two of them are inside every timed step
+    t_d2h = time.perf_counter()
+    for _ in range(3):
+        eng.get_D(Ds)
+    d2h_ms = 1e3 * (time.perf_counter() - t_d2h) / 3
     eng.moments(M)
     alpha_table = eng.alpha_table()  # autodiff reads the kept forward vectors
     eng.autodiff_matrix(cE, Dg)
+    eng.stoch_matrix(cF, Ds)
     kcfg = {name: eng.kernel_config(k) for k, name in
             ((0, "moments"), (1, "xi"), (2, "autodiff"), (3, "outer"), (4, "moments_table"),
              (5, "autodiff_table"))}
     return dict(eng=eng, es=es, N=N, S=S, M=M, mu=mu, energy=energy, wall=wall, times=times,
                 kernel_ms=tot, Dg=Dg.val.copy(), Ds=Ds.val.copy(), config=eng.config(),
                 kernel_config=kcfg, tune_s=t_tune, state=eng.state(), alpha_table=alpha_table,
-                stoch_method=stoch_method,
+                stoch_method=stoch_method, d2h_ms=d2h_ms,
                 energy_scale_s=None if "es" in cfg else t_es)
 
 
@@ -300,7 +316,7 @@ def cpu_baseline(H, groups, cfg, es, seed: int, m_sample: int):
 
 
 def serving_leg(n_req: int, n_max: int, m_max: int, rate: float, seed: int, hbm_gb: int,
-                precs=("fp64", "fp32")) -> dict:
+                precs=("fp64", "fp32"), graph: bool = True) -> dict:
     """The mixed Poisson request stream through the batched engine."""
     rng = np.random.default_rng(seed)
     specs = [serving.draw_spec(rng, n_max=n_max, m_max=m_max) for _ in range(n_req)]
@@ -310,17 +326,18 @@ def serving_leg(n_req: int, n_max: int, m_max: int, rate: float, seed: int, hbm_
                       f"S 16..128, W in [0,6]; each request = moments + autodiff_matrix "
                       f"(energy and dE/dH), HBM arena {hbm_gb} GiB"),
            "updates_total": int(sum(sp.N * sp.M * sp.S for sp in specs))}
+    built = [serving.build_request(sp) for sp in specs]  # host-side H and coefficients, once
     for prec in precs:
-        eng = serving.BatchEngine(prec, hbm_bytes=hbm_gb << 30, max_slots=64)
+        eng = serving.BatchEngine(prec, hbm_bytes=hbm_gb << 30, max_slots=64, graph=graph)
         # first use: kernel loading and the first graph capture
         warm = serving.draw_spec(np.random.default_rng(seed + 1), n_max=8192, m_max=256)
         H, es, c, sd = serving.build_request(warm)
         eng.wait(eng.submit(H, es, c, seeded=sd), want_D=False)
-        recs, wall = serving.run_stream(eng, specs, rate, seed=seed)
+        recs, wall = serving.run_stream(eng, specs, rate, seed=seed, built=built)
         summ = serving.summarize(recs, wall)
         summ["energies"] = [r.energy for _, r in recs]
         # capacity: the same requests all queued at once
-        recs2, wall2 = serving.run_stream(eng, specs, 1e9, seed=seed)
+        recs2, wall2 = serving.run_stream(eng, specs, 1e9, seed=seed, built=built)
         sat = serving.summarize(recs2, wall2)
         summ["saturated"] = {"updates_per_s": sat["updates_per_s"], "wall_s": sat["wall_s"],
                              "p50_ms": sat["p50_ms"], "p99_ms": sat["p99_ms"]}
@@ -370,15 +387,66 @@ def conductivity_leg(H, groups, cfg, es, seed: int, M2: int, reps: int = 2) -> d
     return out
 
 
+def small_solve_leg(name: str, seed: int, reps: int = 20) -> dict:
+    """Wall time of one full solve (moments + autodiff_matrix + stoch_matrix,
+    both D's to the host) of a workload too small to fill the GPU (C1: 1024
+    orbitals x 16 probe columns), where the per-step launches, not HBM, set
+    the time: compared with the sum of the step kernels' device time."""
+    H, groups, cfg = build_workload(name, seed)
+    r = run_gpu(H, groups, cfg, "fp64", reps, 2, seed, tune=False)
+    wall_ms = 1e3 * r["wall"] / reps
+    launches = sum(n for _, n in r["kernel_ms"].values()) // reps
+    dev_ms = sum(ms for ms, _ in r["kernel_ms"].values()) / reps
+    return {"workload": f"{name}: {WORKLOADS[name]}", "N": r["N"], "S": r["S"], "M": r["M"],
+            "solve_ms": round(wall_ms, 3), "step_launches_per_solve": int(launches),
+            "step_loops_device_ms": round(dev_ms, 3),
+            "us_per_step_launch": round(1e3 * dev_ms / max(launches, 1), 2),
+            "energy": r["energy"]}
+
+
+def extra_config_leg(name: str, steps: int, warmup: int, seed: int) -> dict:
+    """FP64 and FP32-mixed steps of another workload (fewer steps than the
+    headline), with per-kernel roofline figures."""
+    H, groups, cfg = build_workload(name, seed)
+    out = {"workload": f"{name}: {WORKLOADS[name]}", "N": H.n_rows * H.b_len, "M": cfg["M"],
+           "steps": steps}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        r = run_gpu(H, groups, cfg, prec, steps, warmup, seed)
+        N, S, M = r["N"], r["S"], r["M"]
+        v = 16 if prec == "fp64" else 8
+        ab = algorithmic_bytes(H, N, S, v,
+                               table_K=(M + 1) // 2 if r["stoch_method"] == "table" else None)
+        kern = {}
+        for k, (ms, n) in r["kernel_ms"].items():
+            if n:
+                us = 1e3 * ms / n
+                kern[k] = {"avg_launch_us": round(us, 3),
+                           "frac": round(ab[k] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+        res[prec] = r
+        out[prec] = {"ms_per_step": round(1e3 * r["wall"] / steps, 3),
+                     "value": N * M * S / (r["wall"] / steps), "kernels": kern,
+                     "stoch_method": r["stoch_method"], "alpha_table": r["alpha_table"],
+                     "d2h_ms_per_D": round(r["d2h_ms"], 3)}
+        out["S"] = S
+    out["fp32_speedup_vs_fp64"] = round(res["fp64"]["wall"] / res["fp32"]["wall"], 3)
+    out["fp32_energy_rel_err_vs_fp64"] = abs(res["fp32"]["energy"] - res["fp64"]["energy"]) / abs(
+        res["fp64"]["energy"])
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="C2")
+    ap.add_argument("--config", default="C3")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-sample-moments", type=int, default=48)
+    ap.add_argument("--cpu-sample-moments", type=int, default=0,
+                    help="M' of the CPU baseline (0: about 5e8 updates of the workload)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the C2 and C1 legs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-mixed comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
@@ -392,6 +460,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-hbm-gb", type=int, default=64)
+    ap.add_argument("--serve-no-graph", action="store_true",
+                    help="serving: launch every tick directly instead of replaying hipGraphs")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -407,7 +477,7 @@ def main(argv=None) -> int:
     if args.serve_only:
         _log("serving stream (full ranges)")
         sv = serving_leg(args.serve_requests, 1 << 20, 2048, args.serve_rate, args.seed,
-                         args.serve_hbm_gb)
+                         args.serve_hbm_gb, graph=not args.serve_no_graph)
         print(json.dumps({"metric": "batched serving: aggregate N·M·S updates/s and p50/p99 "
                                     "latency of a mixed KPM request stream",
                           "value": sv["fp64"]["updates_per_s"], "unit": "updates/s",
@@ -522,38 +592,76 @@ def main(argv=None) -> int:
         "energy_scale_ms": (None if r["energy_scale_s"] is None
                             else round(1e3 * r["energy_scale_s"], 2)),
         "energy": r["energy"],
+        # both D's of a step go to the host inside the timed region, as the
+        # drop-in Engine's calls return them; the same step with D left on the
+        # device would take 2 * d2h_ms_per_D less
+        "pcie": {"d2h_ms_per_D": round(r["d2h_ms"], 3),
+                 "D_bytes": int(H.n_blocks() * H.b_len ** 2 * 16),
+                 "value_device_resident": N * M * S / (r["wall"] / args.steps
+                                                      - 2e-3 * r["d2h_ms"])},
     }
     if not args.no_fp32 and args.precision == "fp64":
         _log("fp32-mixed run")
         r32 = run_gpu(H, groups, cfg, "fp32", args.steps, args.warmup, args.seed)
         t32 = r32["wall"] / args.steps
+        ab32 = algorithmic_bytes(H, N, S, 8, table_K=(M + 1) // 2
+                                 if r32["stoch_method"] == "table" else None)
+        k32 = {}
+        for k, (ms, n) in r32["kernel_ms"].items():
+            if n:
+                us = 1e3 * ms / n
+                k32[k] = {"avg_launch_us": round(us, 3), "bytes_per_launch": int(ab32[k]),
+                          "achieved_GBs": round(ab32[k] / (us * 1e-6) / 1e9, 1),
+                          "frac": round(ab32[k] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                          "ms_per_step": round(ms / args.steps, 3)}
         e_err = abs(r32["energy"] - r["energy"]) / abs(r["energy"])
         dg_err = float(np.max(np.abs(r32["Dg"] - r["Dg"])) / np.max(np.abs(r["Dg"])))
         ds_err = float(np.max(np.abs(r32["Ds"] - r["Ds"])) / np.max(np.abs(r["Ds"])))
         out["fp32_mixed"] = {"value": N * M * S / t32, "ms_per_step": round(1e3 * t32, 3),
                              "speedup_vs_fp64": round(r["wall"] / r32["wall"], 3),
                              "energy_rel_err_vs_fp64": e_err,
-                             "dEdH_rel_err_vs_fp64": dg_err, "Dij_rel_err_vs_fp64": ds_err}
+                             "dEdH_rel_err_vs_fp64": dg_err, "Dij_rel_err_vs_fp64": ds_err,
+                             "kernels": k32, "kernel_config": r32["kernel_config"],
+                             "stoch_method": r32["stoch_method"]}
         del r32
     if not args.no_serve:
         _log("serving stream")
-        out["serving"] = serving_leg(args.serve_requests, 1 << 18, 1024, args.serve_rate,
+        out["serving"] = serving_leg(args.serve_requests, 1 << 20, 2048, args.serve_rate,
                                      args.seed, args.serve_hbm_gb)
-    if not args.no_conductivity and args.precision == "fp64" and "shape" in cfg:
+        sv = out["serving"]
+        out["latency_ms"]["serving_fp64_p50"] = round(sv["fp64"]["p50_ms"], 3)
+        out["latency_ms"]["serving_fp64_p99"] = round(sv["fp64"]["p99_ms"], 3)
+        if "fp32" in sv:
+            out["latency_ms"]["serving_fp32_p50"] = round(sv["fp32"]["p50_ms"], 3)
+            out["latency_ms"]["serving_fp32_p99"] = round(sv["fp32"]["p99_ms"], 3)
+    if not args.no_extras and args.config != "C2":
+        _log("C2 leg")
+        out["C2"] = extra_config_leg("C2", 3, 1, args.seed)
+    if not args.no_extras:
+        _log("C1 solve leg")
+        out["C1"] = small_solve_leg("C1", args.seed)
+    if not args.no_conductivity and args.precision == "fp64":
+        # on the 2-D Kondo lattice (C2), whatever the headline workload
         _log(f"two-current moments, M={args.conductivity_moments}")
-        out["conductivity"] = conductivity_leg(H, groups, cfg, r["es"], args.seed,
+        Hc, gc, cfgc = build_workload("C2", args.seed)
+        ec = HipEngine("cx_double", "fp64")
+        esc = ec.energy_scale(Hc, 0.01)
+        ec.close()
+        out["conductivity"] = conductivity_leg(Hc, gc, cfgc, esc, args.seed,
                                                args.conductivity_moments)
+        out["conductivity"]["workload"] = f"C2: {WORKLOADS['C2']}"
+        del Hc
     _log("exact-diagonalisation check")
     out["exact_check"] = exact_check(args.config, args.precision, args.seed)
     if not args.no_cpu_baseline:
-        _log(f"cpu baseline (oracle, M'={args.cpu_sample_moments})")
-        out["cpu_baseline"] = cpu_baseline(H, groups, cfg, r["es"], args.seed,
-                                           args.cpu_sample_moments)
+        m_cpu = args.cpu_sample_moments or max(2, 2 * round(2.5e8 / (N * S)))
+        _log(f"cpu baseline (oracle, M'={m_cpu})")
+        out["cpu_baseline"] = cpu_baseline(H, groups, cfg, r["es"], args.seed, m_cpu)
         threads = min(16, os.cpu_count() or 1, S)
         if threads > 1:
             _log(f"cpu baseline, {threads} column shards")
             out["cpu_baseline_threads"] = cpu_baseline_threads(
-                H, groups, cfg, r["es"], args.seed, args.cpu_sample_moments, threads)
+                H, groups, cfg, r["es"], args.seed, m_cpu, threads)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -78,20 +78,27 @@ def build_host_lib() -> Path:
     return HOST_LIB
 
 
-CPP_TEST_SRC = ROOT / "tests" / "cpp" / "engine_flow.cpp"
-CPP_TEST_BIN = ROOT / "tests" / "cpp" / "engine_flow"
+CPP_TESTS = ["engine_flow", "ref_adapter"]
 
 
-def build_cpp_test() -> Path:
-    """The C++ application-style test of the fkpm:: API (tests/cpp)."""
-    if not CPP_TEST_SRC.exists():
-        return CPP_TEST_BIN
-    tmp = CPP_TEST_BIN.with_suffix(".tmp")
-    _run_parallel([["g++", "-O2", "-std=c++17", "-Wall", "-I", str(INCLUDE), str(CPP_TEST_SRC),
-                    "-L", str(PKG), "-lfastkpm", "-lfastkpm_hip",
-                    "-Wl,-rpath,$ORIGIN/../../fastkpm_amd", "-o", str(tmp)]])
-    os.replace(tmp, CPP_TEST_BIN)
-    return CPP_TEST_BIN
+def build_cpp_test() -> list[Path]:
+    """The C++ programs of tests/cpp: the application-style flow over the
+    fkpm:: API (engine_flow) and the reference-side Engine<T> that forwards
+    to the C-ABI (ref_adapter)."""
+    cmds, outs = [], []
+    for name in CPP_TESTS:
+        src = ROOT / "tests" / "cpp" / f"{name}.cpp"
+        if not src.exists():
+            continue
+        out = ROOT / "tests" / "cpp" / name
+        cmds.append(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(INCLUDE), str(src),
+                     "-L", str(PKG), "-lfastkpm", "-lfastkpm_hip",
+                     "-Wl,-rpath,$ORIGIN/../../fastkpm_amd", "-o", str(out.with_suffix(".tmp"))])
+        outs.append(out)
+    _run_parallel(cmds)
+    for out in outs:
+        os.replace(out.with_suffix(".tmp"), out)
+    return outs
 
 
 def build_all() -> None:

@@ -532,6 +532,8 @@ struct fkpm_batch {
   std::vector<Slot> active;
   long long next_id = 1;
   bool stop = false;
+  int waiters = 0;  // threads inside fkpm_batch_wait (close waits until none is left)
+  std::condition_variable cv_idle;
   std::string broken;  // sticky failure of the scheduler
   std::thread worker;
   // options and statistics
@@ -551,7 +553,6 @@ struct fkpm_batch {
 
 namespace {
 
-const int kZero = 0;
 
 #define BCHK(x)                                                                       \
   do {                                                                                \
@@ -563,6 +564,19 @@ const int kZero = 0;
       throw std::runtime_error(m_);                                                   \
     }                                                                                 \
   } while (0)
+
+// Host-to-device copy of the first `bytes` of a request's host array; a byte
+// count beyond the array is a planning error and is refused before the copy.
+template <class T>
+void h2d(void* dst, const std::vector<T>& src, size_t bytes, hipStream_t st) {
+  if (bytes > src.size() * sizeof(T)) {
+    char m[160];
+    snprintf(m, sizeof(m), "admission copy of %zu bytes from a host array of %zu bytes", bytes,
+             src.size() * sizeof(T));
+    throw std::runtime_error(m);
+  }
+  if (bytes > 0) BCHK(hipMemcpyAsync(dst, src.data(), bytes, hipMemcpyHostToDevice, st));
+}
 
 void launch_k(BatchKernel k, int grid, hipStream_t st, const SlotDev* sd, const int* stp,
               const int* slots, const int* off, int ns) {
@@ -577,19 +591,18 @@ void admit_one(fkpm_batch* B, Slot& sl) {
   char* base = B->arena.base;
   auto P = [&](int k) { return (void*)(base + L.off[k]); };
   hipStream_t st = B->stream;
-  BCHK(hipMemcpyAsync(P(B_RP), r.rp.data(), L.len[B_RP], hipMemcpyHostToDevice, st));
-  BCHK(hipMemcpyAsync(P(B_CI), r.ci.data(), L.len[B_CI], hipMemcpyHostToDevice, st));
-  BCHK(hipMemcpyAsync(P(B_RB), r.rowblk.data(), L.len[B_RB], hipMemcpyHostToDevice, st));
-  BCHK(hipMemcpyAsync(P(B_TP), r.tpos.data(), L.len[B_TP], hipMemcpyHostToDevice, st));
-  BCHK(hipMemcpyAsync(P(B_HRAW), r.val.data(), L.len[B_HRAW], hipMemcpyHostToDevice, st));
+  h2d(P(B_RP), r.rp, L.len[B_RP], st);
+  h2d(P(B_CI), r.ci, L.len[B_CI], st);
+  h2d(P(B_RB), r.rowblk, L.len[B_RB], st);
+  h2d(P(B_TP), r.tpos, L.len[B_TP], st);
+  h2d(P(B_HRAW), r.val, L.len[B_HRAW], st);
   // compact R: rcol then rval (16-byte aligned) in the staging buffer
   const size_t rv_off = ((size_t)r.n * sizeof(int) + 255) & ~(size_t)255;
   if (r.seeded) {
     // colours and column seeds up; the phases are drawn on the device
     const size_t sd_off = (rv_off + sizeof(double) * 2 * r.n + 255) & ~(size_t)255;
-    BCHK(hipMemcpyAsync(P(B_RRAW), r.rcol.data(), sizeof(int) * r.n, hipMemcpyHostToDevice, st));
-    BCHK(hipMemcpyAsync((char*)P(B_RRAW) + sd_off, r.seeds.data(), sizeof(unsigned) * r.s,
-                        hipMemcpyHostToDevice, st));
+    h2d(P(B_RRAW), r.rcol, sizeof(int) * r.n, st);
+    h2d((char*)P(B_RRAW) + sd_off, r.seeds, sizeof(unsigned) * r.s, st);
     const int* g = (const int*)P(B_RRAW);
     const unsigned* sd = (const unsigned*)((char*)P(B_RRAW) + sd_off);
     double* rv = (double*)((char*)P(B_RRAW) + rv_off);
@@ -597,13 +610,12 @@ void admit_one(fkpm_batch* B, Slot& sl) {
     void* argv[] = {(void*)&g, (void*)&n, (void*)&sd, (void*)&rv};
     BCHK(hipLaunchKernel((const void*)&fkd::k_probe_phases, dim3(r.s), dim3(64), argv, 0, st));
   } else if (r.compact) {
-    BCHK(hipMemcpyAsync(P(B_RRAW), r.rcol.data(), sizeof(int) * r.n, hipMemcpyHostToDevice, st));
-    BCHK(hipMemcpyAsync((char*)P(B_RRAW) + rv_off, r.R.data(), sizeof(double) * 2 * r.n,
-                        hipMemcpyHostToDevice, st));
+    h2d(P(B_RRAW), r.rcol, sizeof(int) * r.n, st);
+    h2d((char*)P(B_RRAW) + rv_off, r.R, sizeof(double) * 2 * r.n, st);
   } else {
-    BCHK(hipMemcpyAsync(P(B_RRAW), r.R.data(), L.len[B_RRAW], hipMemcpyHostToDevice, st));
+    h2d(P(B_RRAW), r.R, L.len[B_RRAW], st);
   }
-  BCHK(hipMemcpyAsync(P(B_C), r.c.data(), L.len[B_C], hipMemcpyHostToDevice, st));
+  h2d(P(B_C), r.c, L.len[B_C], st);
   const double avg = (r.hi + r.lo) / 2, mag = (r.hi - r.lo) / 2;
   if (fkpm_internal::scale_H_cx(st, (const double*)P(B_HRAW), P(B_HV), B->single,
                                 (const int*)P(B_RB), (const int*)P(B_CI), r.nnzb, r.b, mag,
@@ -661,7 +673,10 @@ void admit_one(fkpm_batch* B, Slot& sl) {
   B->h_slots[sl.idx] = d;
   BCHK(hipMemcpyAsync(B->d_slots + sl.idx, &B->h_slots[sl.idx], sizeof(SlotDev),
                       hipMemcpyHostToDevice, st));
-  BCHK(hipMemcpyAsync(B->d_st + sl.idx, &kZero, sizeof(int), hipMemcpyHostToDevice, st));
+  // the slot's step counter starts at 0 (the host mirror holds it; the copy
+  // completes in upload_tables' synchronisation before the mirror is reread)
+  B->h_st[sl.idx] = 0;
+  BCHK(hipMemcpyAsync(B->d_st + sl.idx, &B->h_st[sl.idx], sizeof(int), hipMemcpyHostToDevice, st));
 }
 
 void release_staging(fkpm_batch* B, Slot& sl) {
@@ -741,8 +756,8 @@ void enqueue_ticks(fkpm_batch* B, int T) {
   }
 }
 
-void run_ticks(fkpm_batch* B, int T) {
-  if (!B->use_graph) {
+void run_ticks(fkpm_batch* B, int T, bool use_graph) {
+  if (!use_graph) {
     enqueue_ticks(B, T);
   } else {
     if (B->graph_version != B->set_version) {
@@ -816,15 +831,38 @@ void fail_all(fkpm_batch* B, const std::string& msg) {
   B->cv_done.notify_all();
 }
 
+// Closing: every request still queued or running fails with FKPM_E_STATE and
+// its waiters are woken (called with B->mu held).
+void fail_unfinished_locked(fkpm_batch* B) {
+  for (Slot& sl : B->active) {
+    sl.rq->status = FKPM_E_STATE;
+    sl.rq->err = "the batch engine was closed before the request finished";
+  }
+  for (auto& r : B->pending) {
+    r->status = FKPM_E_STATE;
+    r->err = "the batch engine was closed before the request was admitted";
+  }
+  B->pending.clear();
+  B->cv_done.notify_all();
+}
+
 void scheduler(fkpm_batch* B) {
   try {
     BCHK(hipSetDevice(B->device));
     for (;;) {
       std::vector<size_t> fresh;  // positions in `active` of slots admitted now
+      bool use_graph = true;
+      int max_ticks = 1;
       {
         std::unique_lock<std::mutex> lk(B->mu);
         B->cv_work.wait(lk, [&] { return B->stop || !B->pending.empty() || !B->active.empty(); });
-        if (B->stop) return;
+        if (B->stop) {
+          // the previous iteration synchronised the stream: nothing is in flight
+          fail_unfinished_locked(B);
+          return;
+        }
+        use_graph = B->use_graph;  // options are written under the lock
+        max_ticks = B->max_ticks;
         // admission: FIFO while a slot is free and the footprint fits
         while (!B->pending.empty() && (int)B->active.size() < B->max_slots) {
           std::shared_ptr<Request> r = B->pending.front();
@@ -859,12 +897,12 @@ void scheduler(fkpm_batch* B) {
       }
       if (B->active.empty()) continue;
       // ticks until the first slot finishes (at most max_ticks)
-      int T = B->max_ticks;
+      int T = max_ticks;
       for (const Slot& sl : B->active) {
         const int left = sl.rq->M + 2 - B->h_st[sl.idx];
         T = std::min(T, std::max(left, 1));
       }
-      run_ticks(B, T);
+      run_ticks(B, T, use_graph);
       BCHK(hipStreamSynchronize(B->stream));
       // new slots' staging buffers were consumed by their first ticks
       for (Slot& sl : B->active) release_staging(B, sl);
@@ -974,6 +1012,14 @@ int fkpm_batch_close(fkpm_batch* B) {
   }
   B->cv_work.notify_all();
   if (B->worker.joinable()) B->worker.join();
+  {
+    // the scheduler failed every unfinished request on its way out; wait
+    // until every thread woken in fkpm_batch_wait has left it before the
+    // mutex and condition variables go away
+    std::unique_lock<std::mutex> lk(B->mu);
+    if (B->broken.empty()) fail_unfinished_locked(B);  // scheduler never started its loop
+    B->cv_idle.wait(lk, [&] { return B->waiters == 0; });
+  }
   (void)hipSetDevice(B->device);
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   B->clear_graphs();
@@ -1138,8 +1184,10 @@ int fkpm_batch_wait(fkpm_batch* B, long long id, double* mu, double* energy, dou
     auto it = B->reqs.find(id);
     if (it == B->reqs.end()) return berr(FKPM_E_ARG, "unknown request %lld", id);
     r = it->second;
+    B->waiters++;
     B->cv_done.wait(lk, [&] { return r->status == 2 || r->status < 0; });
-    B->reqs.erase(it);
+    B->reqs.erase(r->id);
+    if (--B->waiters == 0) B->cv_idle.notify_all();
   }
   if (r->status < 0) return berr(r->status, "request %lld failed: %s", id, r->err.c_str());
   if (mu) std::copy(r->mu.begin(), r->mu.end(), mu);

@@ -160,6 +160,9 @@ struct KCfg {
   int nwg = 0, chunk = 0;
 };
 
+// Most blocks per row of the row-resident matrix-core kernels (fkd::MF_DM).
+constexpr int kRowResidentMax = 8;
+
 template <class S>
 fkd::RSource<S> no_rsource() {
   fkd::RSource<S> r;
@@ -212,6 +215,8 @@ struct fkpm_handle {
   std::vector<double> tabc_host;
   int store_alpha = 1;    // FKPM_OPT_STORE_ALPHA: 0 never, 1 auto, 2 whenever the table fits
   int sweep_force = -1;  // FKPM_OPT_SWEEP: row sweep of every kind (-1: per configuration)
+  int slice_force = -1;  // FKPM_OPT_SLICE: column-pass loop order (-1: per configuration)
+  long long table_cap_mb = -1;  // FKPM_OPT_ALPHA_TABLE_MB (-1: the default budget)
 
   // --- D ---
   int nbD = 0, nnzbD = 0;
@@ -307,10 +312,10 @@ bool mfma_avail(const fkpm_handle* h, int kind) {
   return h->mfma_on && h->cx && h->b == 8 && (kind == K_MOM || kind == K_MOMT || kind == K_AD);
 }
 
-// The row-resident variants: rows of at most 8 blocks, byte offsets of a
-// vector within 32 bits.
+// The row-resident variants: rows of at most kRowResidentMax blocks (MF_DM of
+// kpm_mfma.h), byte offsets of a vector within 32 bits.
 bool mfma_rr_avail(const fkpm_handle* h, int kind) {
-  return mfma_avail(h, kind) && h->max_deg <= 8 && h->vbytes() < (size_t(1) << 32);
+  return mfma_avail(h, kind) && h->max_deg <= kRowResidentMax && h->vbytes() < (size_t(1) << 32);
 }
 
 int set_kind(fkpm_handle* h, int kind, KCfg c) {
@@ -324,6 +329,8 @@ int set_kind(fkpm_handle* h, int kind, KCfg c) {
   if (!lookup_kset(h, kind, h->b, c.L, c.KQ))
     return fail(FKPM_E_UNSUP, "no kernel for block %d, lanes %d, packets %d", h->b, c.L, c.KQ);
   if (h->sweep_force >= 0) c.sweep = h->sweep_force;
+  if (h->slice_force >= 0 && !c.mfma)
+    c.nt = (c.nt & ~fkd::NT_SLICE) | (h->slice_force ? fkd::NT_SLICE : 0);
   derive_grid(h->nb, c);
   h->kc[kind] = c;
   return 0;
@@ -400,7 +407,7 @@ std::map<std::string, std::array<KCfg, NKIND>> g_tune_cache;
 std::string tune_key(const fkpm_handle* h) {
   char k[128];
   snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb,
-           h->nnzb, h->mfma_on ? 1 : 0, h->max_deg <= 8 ? 1 : 0, table_wanted(h) ? 1 : 0);
+           h->nnzb, h->mfma_on ? 1 : 0, h->max_deg <= kRowResidentMax ? 1 : 0, table_wanted(h) ? 1 : 0);
   return k;
 }
 
@@ -512,7 +519,12 @@ int ensure_ready(fkpm_handle* h) {
   if (h->n != h->nb * h->b)
     return fail(FKPM_E_ARG, "R has %d rows but H has %d = %d blocks x %d", h->n, h->nb * h->b,
                 h->nb, h->b);
-  if (!h->r_dirty && h->geo_b == h->b) return 0;
+  bool stale = h->r_dirty || h->geo_b != h->b;
+  // backstop: a row-resident matrix-core kernel on a pattern with longer rows
+  // would read only the first kRowResidentMax blocks of each row
+  for (int k = 0; k < NKIND && !stale; k++)
+    stale = h->kc[k].mfma == 2 && !mfma_rr_avail(h, k);
+  if (!stale) return 0;
   CHK(choose_geometry(h));
   const size_t vb = h->vbytes();
   CHK(h->R.ensure(vb));
@@ -640,17 +652,21 @@ int tune_t(fkpm_handle* h) {
     for (int g = 0; g < ng; g++) {
       const int w = gs[g].L * gs[g].KQ;
       if (w > h->wmax || 2 * w < h->wmax) continue;
+      // the column passes as the outer loop (NT_SLICE) when a row takes
+      // several passes of this width
+      const int nsl = h->qp / w >= 2 ? 2 : 1;
       for (int cap : caps)
         for (int nt : nts)
-          for (int sw : sweeps) {
-            KCfg c;
-            c.L = gs[g].L;
-            c.KQ = gs[g].KQ;
-            c.maxwg = cap;
-            c.nt = nt;
-            c.sweep = sw;
-            cands.push_back(c);
-          }
+          for (int sl = 0; sl < nsl; sl++)
+            for (int sw : sweeps) {
+              KCfg c;
+              c.L = gs[g].L;
+              c.KQ = gs[g].KQ;
+              c.maxwg = cap;
+              c.nt = nt | (sl ? fkd::NT_SLICE : 0);
+              c.sweep = sw;
+              cands.push_back(c);
+            }
     }
     if (mfma_avail(h, kind))
       for (int v = 1; v <= (mfma_rr_avail(h, kind) ? 2 : 1); v++)
@@ -757,7 +773,11 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
     const size_t need = (size_t)nsteps * h->vbytes();
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    size_t budget = free_b / 2 + h->atab.bytes;
+    // default: half of what is free (the table already held counts as free),
+    // never more than a quarter of the device, so that other engines and a
+    // batch arena on the same device keep room whatever the order of calls
+    size_t budget = std::min(free_b / 2 + h->atab.bytes, total_b / 4);
+    if (h->table_cap_mb >= 0) budget = (size_t)h->table_cap_mb << 20;
     if (const char* e = getenv("FKPM_ALPHA_TABLE_BYTES")) budget = (size_t)strtoull(e, nullptr, 10);
     h->a_stored = false;
     if (table_wanted(h) && !h->kc[K_AD].mfma && need <= budget) {
@@ -1615,8 +1635,14 @@ int fkpm_set_H(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, cons
     for (int p = row_ptr[i]; p < row_ptr[i + 1]; p++) found |= (col_idx[p] == i);
     if (!found) return fail(FKPM_E_ARG, "set_H: diagonal block (%d,%d) is missing", i, i);
   }
-  const bool geo_change = (b != h->b);
+  // The kernel choice depends on the block size, the block-row and block
+  // counts (grids, tuned-shape key) and on whether every row has at most
+  // kRowResidentMax blocks (row-resident matrix-core kernels read only that many): a
+  // change of any of them picks the kernels again at the next use.
+  const int old_b = h->b, old_nb = h->nb, old_nnzb = h->nnzb, old_deg = h->max_deg;
   CHK(upload_pattern(h, nb, b, nnzb, row_ptr, col_idx));
+  const bool geo_change = b != old_b || nb != old_nb || nnzb != old_nnzb ||
+                          (old_deg <= kRowResidentMax) != (h->max_deg <= kRowResidentMax);
   h->avg = avg;
   h->mag = mag;
   const size_t eb = h->single ? 4 : 8;
@@ -1913,6 +1939,18 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
             h->kc[k].sweep = value;
             derive_grid(h->nb, h->kc[k]);
           }
+      return 0;
+    case FKPM_OPT_SLICE:
+      if (value < -1 || value > 1) return fail(FKPM_E_ARG, "bad slice switch %d", value);
+      h->slice_force = value;
+      if (value >= 0)
+        for (int k = 0; k < NKIND; k++)
+          if (h->kc[k].L > 0 && !h->kc[k].mfma)
+            h->kc[k].nt = (h->kc[k].nt & ~fkd::NT_SLICE) | (value ? fkd::NT_SLICE : 0);
+      return 0;
+    case FKPM_OPT_ALPHA_TABLE_MB:
+      if (value < -1) return fail(FKPM_E_ARG, "bad table budget %d MiB", value);
+      h->table_cap_mb = value;
       return 0;
     case FKPM_OPT_MFMA:
       if (value != 0 && value != 1) return fail(FKPM_E_ARG, "bad MFMA switch %d", value);

@@ -112,6 +112,13 @@ __device__ __forceinline__ void stv_nt(T* __restrict__ p, const A* v, bool nt) {
 // Bits of the `nt` kernel argument.
 constexpr int NT_LOAD = 1;   // loads of the operands read once per step
 constexpr int NT_STORE = 2;  // stores of the step's outputs
+// Loop order of the step kernels: without NT_SLICE a lane group finishes
+// every column pass of a block row before the next row; with it the
+// workgroups sweep all their rows once per column pass (L*KQ packets), so the
+// rows one XCD touches between a row and its far neighbours (a 3-D lattice's
+// next plane) take 1/passes of the bytes and stay in its L2, at the cost of
+// reading H's indices and values once per pass.
+constexpr int NT_SLICE = 4;
 
 // DPP lane move inside 16-lane rows (gfx9 encodings: quad_perm 0x00-0xFF,
 // row_mirror 0x140, row_half_mirror 0x141).
@@ -574,6 +581,8 @@ __device__ __forceinline__ void step_moments_dev(
   double dmm = 0.0, dm1 = 0.0;
   const S* const X[1] = {x};
 
+  const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
+  for (int qs = 0; qs < qp; qs += qw) {
   RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
@@ -582,7 +591,7 @@ __device__ __forceinline__ void step_moments_dev(
     nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
     const int p0 = cur.p0, p1 = cur.p1;
-    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+    for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A xd[KQ][B][NS];
       A acc[1][KQ][B][NS];
@@ -643,6 +652,7 @@ __device__ __forceinline__ void step_moments_dev(
       }
     }
   }
+  }
   if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
 }
 
@@ -676,6 +686,8 @@ __global__ __launch_bounds__(WG) void k_step_xi(
   const A acx = (A)cx, acz = (A)cz;
   const S* const X[1] = {x};
 
+  const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
+  for (int qs = 0; qs < qp; qs += qw) {
   RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
@@ -684,7 +696,7 @@ __global__ __launch_bounds__(WG) void k_step_xi(
     nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
     const int p0 = cur.p0, p1 = cur.p1;
-    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+    for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A acc[1][KQ][B][NS];
 #pragma unroll
@@ -741,6 +753,7 @@ __global__ __launch_bounds__(WG) void k_step_xi(
         }
       }
     }
+  }
   }
 }
 
@@ -827,6 +840,8 @@ __device__ __forceinline__ void step_autodiff_dev(
   const double fD = dscale * hb1;
   using DPre = decltype(load_dpre<S, CX, B, L, KQ, FUSED_D>(dacc, 0, 0));
 
+  const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
+  for (int qs = 0; qs < qp; qs += qw) {
   RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
@@ -835,7 +850,7 @@ __device__ __forceinline__ void step_autodiff_dev(
     nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
     const int p0 = cur.p0, p1 = cur.p1;
-    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+    for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A ai[KQ][B][NS];
       A acc[2][KQ][B][NS];
@@ -961,6 +976,7 @@ __device__ __forceinline__ void step_autodiff_dev(
       }
     }
   }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1001,6 +1017,8 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
   using DPre = decltype(load_dpre<S, CX, B, L, KQ, FUSED_D>(dacc, 0, 0));
   const A none[KQ][B][NS] = {};
 
+  const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
+  for (int qs = 0; qs < qp; qs += qw) {
   RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
@@ -1009,7 +1027,7 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
     nxt = row_ptr(rp, i + rr.stride, rr.r1);
     if (i >= rr.r1) continue;
     const int p0 = cur.p0, p1 = cur.p1;
-    for (int q0 = 0; q0 < qp; q0 += L * KQ) {
+    for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
       A ai[KQ][B][NS];
       A acc[1][KQ][B][NS];
@@ -1128,6 +1146,7 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
         }
       }
     }
+  }
   }
 }
 

@@ -310,6 +310,18 @@ class HipEngine:
         configuration's own (FKPM_OPT_SWEEP)."""
         call("fkpm_set_option", self._h, _capi.OPT_SWEEP, int(sweep))
 
+    def set_slice(self, mode: int) -> None:
+        """Column-pass loop order of the vector-unit step kernels
+        (FKPM_OPT_SLICE): 1 every row once per column pass, 0 a row's passes
+        back to back, -1 the configuration's own."""
+        call("fkpm_set_option", self._h, _capi.OPT_SLICE, int(mode))
+
+    def set_alpha_table_mb(self, mib: int) -> None:
+        """Most MiB the table of forward vectors may take on this handle
+        (FKPM_OPT_ALPHA_TABLE_MB; -1: a quarter of the device, at most half
+        of what is free)."""
+        call("fkpm_set_option", self._h, _capi.OPT_ALPHA_TABLE_MB, int(mib))
+
     def set_mfma(self, on: bool) -> None:
         """Matrix-core kernels where the block shape has them (complex 8x8
         blocks; on by default), or the vector-unit kernels only."""

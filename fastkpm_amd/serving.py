@@ -226,13 +226,15 @@ def build_request(spec: RequestSpec, kT: float = 0.05, mu: float = 0.0):
 
 
 def run_stream(eng: BatchEngine, specs, rate_hz: float, seed: int = 0, want_D: bool = False,
-               log=None):
+               log=None, built=None):
     """Submit the requests at Poisson arrival times (rate_hz per second, built
-    ahead of time), then collect every result.  Returns per-request records and
-    the wall time from the first arrival to the last completion."""
+    ahead of time, or given as `built`, the build_request of each spec), then
+    collect every result.  Returns per-request records and the wall time from
+    the first arrival to the last completion."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, len(specs))
-    built = [build_request(s) for s in specs]
+    if built is None:
+        built = [build_request(s) for s in specs]
     t0 = time.perf_counter()
     arrival = t0 + np.cumsum(gaps) - gaps[0]
     ids = []

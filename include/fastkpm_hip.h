@@ -172,6 +172,19 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
                                    1..64: as one front in segments of s passes per XCD
                                    (fkpm_get_kernel_sweep); -1 (default): the configuration's
                                    own (the tuner's choice, else one front with s = 2) */
+#define FKPM_OPT_SLICE 6        /* value 1: the vector-unit step kernels sweep all their block
+                                   rows once per column pass (the probe columns of a row in
+                                   slices of one pass width), so the rows an XCD gathers
+                                   between a row and its far neighbours take fewer bytes of
+                                   its L2; 0: a row's column passes back to back; -1
+                                   (default): the configuration's own (the tuner's choice,
+                                   else 0) */
+#define FKPM_OPT_ALPHA_TABLE_MB 7 /* value >= 0: the most MiB the table of forward vectors
+                                   (FKPM_OPT_STORE_ALPHA) may hold on this handle; -1
+                                   (default): a quarter of the device's memory and at most
+                                   half of what is free when fkpm_moments runs (the
+                                   environment variable FKPM_ALPHA_TABLE_BYTES, read at each
+                                   call, takes precedence over both) */
 #define FKPM_STOCH_CLENSHAW 0   /* xi by the Clenshaw recurrence: one 2-vector pass per order */
 #define FKPM_STOCH_FORWARD 1    /* xi accumulated in FP64 along the forward recursion, the
                                    reference's summation order (engine_cpu.cpp:236-244) */
@@ -258,6 +271,11 @@ int fkpm_batch_footprint(int prec, int nb, int b, int nnzb, int s, int M, size_t
 /* Open a batched engine on `device` with an arena of `hbm_bytes` and at most
  * `max_slots` requests in flight; prec = FKPM_PREC_FP64 or FKPM_PREC_FP32. */
 int fkpm_batch_open(int device, int prec, size_t hbm_bytes, int max_slots, fkpm_batch** out);
+/* Stop the scheduler, fail every request still queued or running with
+ * FKPM_E_STATE (threads blocked in fkpm_batch_wait on them return that code),
+ * wait until those threads have left fkpm_batch_wait, then release the
+ * device memory.  No other call may start on the handle once close has
+ * begun. */
 int fkpm_batch_close(fkpm_batch* b);
 int fkpm_batch_set_option(fkpm_batch* b, int option, int value);
 

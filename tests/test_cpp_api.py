@@ -74,3 +74,48 @@ def test_cpp_api_device_flow_matches_oracle(oracle, tmp_path, autotune):
     c2 = host.conductivity_coefficients(M2, 2 * M2, 0.05, 0.0, 0.0, lo, hi)
     s_o = complex(np.sum(c2 * mu2_o))
     assert abs(sigma - s_o) < 1e-10 * abs(s_o)
+
+
+ADAPTER = Path(__file__).resolve().parent / "cpp" / "ref_adapter"
+
+# element type -> (numpy type of the dumped values, tolerance of the D outputs):
+# the adapter opens the C-ABI in FP64 for every element type, so moments agree
+# with the oracle to the FP64 bar; single-precision D values are rounded once
+# to float on the way out (2^-24 relative per element).
+ADAPTER_TYPES = {"cx_double": (np.complex128, 1e-10), "double": (np.float64, 1e-10),
+                 "cx_float": (np.complex64, 1e-7), "float": (np.float32, 1e-7)}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("etype", list(ADAPTER_TYPES))
+def test_reference_side_adapter_matches_oracle(oracle, tmp_path, etype):
+    """tests/cpp/ref_adapter.cpp: an fkpm::Engine<T> forwarding to the C-ABI the
+    way a maintainer of the reference would bind it (block-row counts passed as
+    SpMatBsr::n_rows), on a 12 x 12 two-orbital lattice (b = 2) with correlated
+    probing R, against the oracle on the same H and R."""
+    if not ADAPTER.exists():
+        pytest.fail(f"{ADAPTER} is not built (run __graft_entry__.build())")
+    r = subprocess.run([str(ADAPTER), etype, str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from fastkpm_amd import SpMatBsr
+    dt, tol = ADAPTER_TYPES[etype]
+    meta = (tmp_path / "meta.txt").read_text().split()
+    n, s, M, nb, nnzb = (int(v) for v in meta[:5])
+    lo, hi = (float(v) for v in meta[5:7])
+    rd = lambda name, t: np.fromfile(tmp_path / name, dtype=t)
+    H = SpMatBsr(nb, nb, 2, rd("H_row_idx.bin", np.int32), rd("H_col_idx.bin", np.int32),
+                 rd("H_row_ptr.bin", np.int32), rd("H_val.bin", dt))
+    assert H.n_blocks() == nnzb and n == 2 * nb
+    R = rd("R.bin", dt).reshape(s, n).T
+    assert s == 8 and np.all(np.count_nonzero(R, axis=1) == 1)
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(R)
+    mu_o = orc.moments(M)
+    assert rel_err(rd("mu.bin", np.float64), mu_o) < 1e-10
+    assert mu_o[0] == pytest.approx(n, rel=1e-12)
+    d_o = orc.autodiff_matrix(rd("cE.bin", np.float64), H)
+    assert rel_err(rd("D_autodiff.bin", dt), d_o) < tol
+    s_o = orc.stoch_matrix(rd("cF.bin", np.float64), H)
+    assert rel_err(rd("D_stoch.bin", dt), s_o) < tol

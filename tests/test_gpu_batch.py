@@ -234,3 +234,38 @@ def test_batch_refusals(oracle):
         eng.submit(H2, EnergyScale(-6, 6), np.ones(4),
                    seeded=(np.arange(32), np.arange(32, dtype=np.uint32)))
     eng.close()
+
+
+def test_batch_close_wakes_waiters():
+    """Closing the engine while other threads wait on a running and on a queued
+    request: each waiter returns (the result, or FKPM_E_STATE for a request
+    the engine did not finish) and close returns after them."""
+    import threading
+    from fastkpm_amd import serving
+    from fastkpm_amd._capi import FkpmError
+    big = serving.RequestSpec(3, 2, 40, 4, 1024, 2.0, 17)  # N = 128000, S = 128, M = 1024
+    H, es, c, sd = serving.build_request(big)
+    fp = serving.footprint("fp64", H, len(sd[1]), len(c))
+    # room for one of them at a time: the second stays queued
+    eng = serving.BatchEngine("fp64", hbm_bytes=int(fp * 1.5), max_slots=4)
+    ids = [eng.submit(H, es, c, seeded=sd) for _ in range(2)]
+    out = {}
+
+    def waiter(k, rid):
+        try:
+            out[k] = eng.wait(rid, want_D=False).energy
+        except FkpmError as e:
+            out[k] = e
+
+    threads = [threading.Thread(target=waiter, args=(k, rid)) for k, rid in enumerate(ids)]
+    for t in threads:
+        t.start()
+    import time
+    time.sleep(0.3)  # both waiters inside fkpm_batch_wait, the first request running
+    eng.close()
+    for t in threads:
+        t.join(timeout=60)
+        assert not t.is_alive()
+    assert len(out) == 2
+    # the queued request cannot have finished: the arena held one at a time
+    assert any(isinstance(v, FkpmError) and v.code == -2 for v in out.values()), out

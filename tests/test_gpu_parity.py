@@ -420,6 +420,38 @@ def test_every_compiled_geometry(oracle):
         eng.set_geometry(8, 1, 0)  # not compiled for b = 2
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_column_slices_outer_loop(oracle, prec):
+    """FKPM_OPT_SLICE 1: every step kernel sweeps its rows once per column
+    pass (S = 40 probe columns: several passes for every geometry below), for
+    the forward step, the Clenshaw and forward-order stoch_matrix, and the
+    autodiff step with alpha recomputed and read from the kept table."""
+    H, R, es, orc = _pair(oracle, J=1.0, uncorrelated_S=40)
+    M = 40
+    c = fermi_coeffs(oracle, M, es.lo, es.hi)
+    mu_o = orc.moments(M)
+    d_o = orc.autodiff_matrix(c, H)
+    s_o = orc.stoch_matrix(c, H)
+    tol_mu, tol_d = (FP64_TOL, FP64_TOL) if prec == "fp64" else (2e-5, 1e-4)
+    for L, KQ in [(4, 1), (8, 2), (16, 2), (64, 1)]:
+        for store, method in ((0, "clenshaw"), (2, "auto"), (0, "forward")):
+            eng = _engine("cx_double", prec)
+            eng.set_R(R)
+            eng.set_H(H, es)
+            eng.set_geometry(L, KQ, 0)
+            eng.set_slice(1)
+            eng.set_store_alpha(store)
+            eng.set_stoch_method(method)
+            assert rel_err(eng.moments(M), mu_o) < tol_mu, (L, KQ, store)
+            assert eng.kernel_config(0)["nt"] & 4
+            D = H.like()
+            eng.autodiff_matrix(c, D)
+            assert rel_err(D.val, d_o) < tol_d, (L, KQ, store)
+            eng.stoch_matrix(c, D)
+            assert rel_err(D.val, s_o) < tol_d, (L, KQ, method)
+            eng.close()
+
+
 @pytest.mark.parametrize("prec,store", [("fp64", 1), ("fp32", 1), ("fp64", 2), ("fp32", 2)])
 def test_autotuned_engine(oracle, prec, store):
     """store 2: the forward vectors are kept, and the tuner also times the
