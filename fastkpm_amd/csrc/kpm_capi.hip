@@ -61,15 +61,106 @@ struct TypeId {
   typedef T type;
 };
 
+// One recorded kernel launch: entry, grid, block and the argument bytes.
+struct LaunchRec {
+  const void* fn = nullptr;
+  dim3 grid, block;
+  std::vector<size_t> offs;
+  std::vector<unsigned char> args;
+};
+
+// While a recorder is active on this thread for a stream, launch() on that
+// stream appends to it instead of launching (LaunchRecorder below replays the
+// run directly or as a cached hipGraph).
+struct RecSink {
+  hipStream_t stream = nullptr;
+  std::vector<LaunchRec>* recs = nullptr;
+};
+thread_local RecSink g_rec;
+
 // Launch a kernel through its host stub with exactly its parameter types.
 template <typename... P>
 int launch(void (*fp)(P...), dim3 grid, dim3 block, hipStream_t st,
            typename TypeId<P>::type... args) {
   void* argv[] = {(void*)&args...};
+  if (g_rec.recs != nullptr && g_rec.stream == st) {
+    static constexpr size_t sz[] = {sizeof(P)...};
+    static constexpr size_t al[] = {alignof(P)...};
+    LaunchRec r;
+    r.fn = (const void*)fp;
+    r.grid = grid;
+    r.block = block;
+    size_t off = 0;
+    for (size_t k = 0; k < sizeof...(P); k++) {
+      off = (off + al[k] - 1) / al[k] * al[k];
+      r.offs.push_back(off);
+      off += sz[k];
+    }
+    r.args.resize(off);
+    for (size_t k = 0; k < sizeof...(P); k++) memcpy(r.args.data() + r.offs[k], argv[k], sz[k]);
+    g_rec.recs->push_back(std::move(r));
+    return 0;
+  }
   hipError_t e = hipLaunchKernel((const void*)fp, grid, block, argv, 0, st);
   if (e != hipSuccess) return fail(FKPM_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return 0;
 }
+
+int replay_direct(const std::vector<LaunchRec>& recs, hipStream_t st) {
+  std::vector<void*> argv;
+  for (const LaunchRec& r : recs) {
+    argv.resize(r.offs.size());
+    for (size_t k = 0; k < r.offs.size(); k++)
+      argv[k] = const_cast<unsigned char*>(r.args.data() + r.offs[k]);
+    hipError_t e = hipLaunchKernel(r.fn, r.grid, r.block, argv.data(), 0, st);
+    if (e != hipSuccess) return fail(FKPM_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+  }
+  return 0;
+}
+
+// Identity of a recorded run: every launch's entry, grid and argument bytes.
+uint64_t rec_hash(const std::vector<LaunchRec>& recs) {
+  uint64_t x = 1469598103934665603ull;  // FNV-1a
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; i++) x = (x ^ b[i]) * 1099511628211ull;
+  };
+  for (const LaunchRec& r : recs) {
+    mix(&r.fn, sizeof(r.fn));
+    const unsigned g[6] = {r.grid.x, r.grid.y, r.grid.z, r.block.x, r.block.y, r.block.z};
+    mix(g, sizeof(g));
+    mix(r.args.data(), r.args.size());
+  }
+  return x;
+}
+
+bool rec_equal(const std::vector<LaunchRec>& a, const std::vector<LaunchRec>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t k = 0; k < a.size(); k++) {
+    const LaunchRec &x = a[k], &y = b[k];
+    if (x.fn != y.fn || x.grid.x != y.grid.x || x.grid.y != y.grid.y || x.grid.z != y.grid.z ||
+        x.block.x != y.block.x || x.block.y != y.block.y || x.block.z != y.block.z ||
+        x.args != y.args)
+      return false;
+  }
+  return true;
+}
+
+// Instantiated graphs of recorded runs, most recently used first.
+struct GraphCache {
+  struct Entry {
+    uint64_t key;
+    std::vector<LaunchRec> recs;
+    hipGraphExec_t exec;
+  };
+  std::vector<Entry> entries;
+  static constexpr size_t kMax = 16;
+  void clear() {
+    for (Entry& e : entries) (void)hipGraphExecDestroy(e.exec);
+    entries.clear();
+  }
+  ~GraphCache() { clear(); }
+};
 
 struct DevBuf {
   void* p = nullptr;
@@ -227,6 +318,11 @@ struct fkpm_handle {
   // --- two-current moments (moments2) ---
   DevBuf j_rp[2], j_ci[2], j_rb[2], j_raw[2], j_val[2], m2_a, m2_b, m2_ring, m2_c, m2_part;
 
+  // --- hipGraph replay of the step loops (short steps) ---
+  int graph_mode = -1;  // FKPM_OPT_GRAPH: -1 automatic, 0 never, 1 always
+  GraphCache gcache;
+  long long n_graph_launches = 0, n_graph_captures = 0;
+
   // --- timing ---
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -244,6 +340,7 @@ struct fkpm_handle {
     DevBuf* m2bufs[] = {&j_rp[0], &j_ci[0], &j_rb[0], &j_raw[0], &j_val[0], &j_rp[1], &j_ci[1],
                         &j_rb[1], &j_raw[1], &j_val[1], &m2_a, &m2_b, &m2_ring, &m2_c, &m2_part};
     for (DevBuf* d : m2bufs) d->release();
+    gcache.clear();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -273,6 +370,73 @@ const fkd::KSet<float>& fkpm_handle::ks<float>(int kind) const {
 }
 
 namespace {
+
+// Vectors of at most this many scalars count as short steps: their launches,
+// not HBM, set the time of a step loop, so the loop is replayed as a hipGraph.
+constexpr size_t kGraphMaxScalars = (size_t)1 << 22;
+
+// Records the kernel launches of one step loop on the handle's stream; run()
+// then launches them directly or, for short steps (FKPM_OPT_GRAPH), replays
+// the cached hipGraph of exactly the same launches, capturing it the first
+// time.  A graph replays launches with identical entries, grids and argument
+// bytes, so it computes what the direct launches compute.
+class LoopRecorder {
+ public:
+  explicit LoopRecorder(fkpm_handle* h) : h_(h) {
+    g_rec.stream = h->stream;
+    g_rec.recs = &recs_;
+  }
+  ~LoopRecorder() { stop(); }
+  int run() {
+    stop();
+    const bool graph = h_->graph_mode == 1 ||
+                       (h_->graph_mode == -1 && recs_.size() >= 8 &&
+                        (size_t)h_->n * h_->sp * (h_->cx ? 2 : 1) <= kGraphMaxScalars);
+    if (!graph) return replay_direct(recs_, h_->stream);
+    GraphCache& gc = h_->gcache;
+    const uint64_t key = rec_hash(recs_);
+    for (size_t k = 0; k < gc.entries.size(); k++) {
+      if (gc.entries[k].key == key && rec_equal(gc.entries[k].recs, recs_)) {
+        std::rotate(gc.entries.begin(), gc.entries.begin() + k, gc.entries.begin() + k + 1);
+        HIPCHK(hipGraphLaunch(gc.entries[0].exec, h_->stream));
+        h_->n_graph_launches++;
+        return 0;
+      }
+    }
+    HIPCHK(hipStreamBeginCapture(h_->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = replay_direct(recs_, h_->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h_->stream, &g);
+    if (rc != 0 || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc != 0 ? rc : fail(FKPM_E_HIP, "graph capture: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(FKPM_E_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+    if (gc.entries.size() >= GraphCache::kMax) {
+      HIPCHK(hipStreamSynchronize(h_->stream));  // no replay of the evicted graph in flight
+      (void)hipGraphExecDestroy(gc.entries.back().exec);
+      gc.entries.pop_back();
+    }
+    gc.entries.insert(gc.entries.begin(), GraphCache::Entry{key, std::move(recs_), ex});
+    h_->n_graph_captures++;
+    HIPCHK(hipGraphLaunch(ex, h_->stream));
+    h_->n_graph_launches++;
+    return 0;
+  }
+
+ private:
+  void stop() {
+    if (g_rec.recs == &recs_) {
+      g_rec.recs = nullptr;
+      g_rec.stream = nullptr;
+    }
+  }
+  fkpm_handle* h_;
+  std::vector<LaunchRec> recs_;
+};
 
 bool lookup_kset(fkpm_handle* h, int kind, int B, int L, int KQ) {
   if (h->single)
@@ -791,20 +955,24 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
   const auto& ks = h->ks<S>(kind);
   const dim3 grid(kc.nwg), block(fkd::WG);
   CHK(timing_begin(h));
-  for (int m = 0; m < nsteps; m++) {
-    // alpha_{m+1} = cx Hs alpha_m + cz alpha_{m-1}
-    const S* x = (const S*)h->vec_a(m);
-    const S* z = m == 0 ? nullptr : (const S*)h->vec_a(m - 1);
-    S* y = (S*)h->vec_a(m + 1);
-    CHK(launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
-               h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk, m == 0 ? 1.0 : 2.0,
-               m == 0 ? 0.0 : -1.0, h->partial.as<double>() + (size_t)m * kc.nwg * 2,
-               kc.nt, no_rsource<S>()));
+  {
+    LoopRecorder rec(h);
+    for (int m = 0; m < nsteps; m++) {
+      // alpha_{m+1} = cx Hs alpha_m + cz alpha_{m-1}
+      const S* x = (const S*)h->vec_a(m);
+      const S* z = m == 0 ? nullptr : (const S*)h->vec_a(m - 1);
+      S* y = (S*)h->vec_a(m + 1);
+      CHK(launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(),
+                 h->ci.as<const int>(), h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk,
+                 m == 0 ? 1.0 : 2.0, m == 0 ? 0.0 : -1.0,
+                 h->partial.as<double>() + (size_t)m * kc.nwg * 2, kc.nt, no_rsource<S>()));
+    }
+    CHK(launch(&fkd::k_reduce_partials, dim3(nsteps), block, h->stream,
+               (const double*)h->partial.as<double>(), kc.nwg, h->psum.as<double>()));
+    CHK(rec.run());
   }
   CHK(timing_end(h, &h->ms_mom, &h->n_mom, nsteps));
   if (h->a_stored) h->tab_K = nsteps;
-  CHK(launch(&fkd::k_reduce_partials, dim3(nsteps), block, h->stream,
-             (const double*)h->partial.as<double>(), kc.nwg, h->psum.as<double>()));
   std::vector<double> ps((size_t)nsteps * 2);
   HIPCHK(hipMemcpyAsync(ps.data(), h->psum.p, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
                         h->stream));
@@ -963,8 +1131,13 @@ int stoch_t(fkpm_handle* h, int M, const double* c) {
                                                                 : FKPM_STOCH_CLENSHAW;
   if (h->stoch_used != FKPM_STOCH_FORWARD) {
     const S* xi = nullptr;
-    const int launches = table ? stoch_table<S>(h, M, c, &xi) : stoch_clenshaw<S>(h, M, c, &xi);
-    if (launches < 0) return launches;
+    int launches = 0;
+    {
+      LoopRecorder rec(h);
+      launches = table ? stoch_table<S>(h, M, c, &xi) : stoch_clenshaw<S>(h, M, c, &xi);
+      if (launches < 0) return launches;
+      CHK(rec.run());
+    }
     CHK(timing_end(h, &h->ms_stoch, &h->n_stoch, launches));
     CHK(zero_doubles(h, h->dacc.p, dcount));
     // D = (1/2) R xi^+ + (1/2) xi R^+
@@ -980,15 +1153,19 @@ int stoch_t(fkpm_handle* h, int M, const double* c) {
     const KCfg& kx = h->kc[K_XI];
     const auto& ksx = h->ks<S>(K_XI);
     const dim3 grid(kx.nwg);
-    // alpha_1 = Hs R, xi = c0 R + c1 alpha_1
-    CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(0),
-               (const S*)nullptr, (S*)h->vec_a(1), h->xi.as<double>(), h->nb, h->qp, kx.chunk, 1.0,
-               0.0, c[0], c[1], 1, kx.nt));
-    for (int m = 2; m < M; m++) {
-      // alpha_m = 2 Hs alpha_{m-1} - alpha_{m-2}; xi += c_m alpha_m
-      CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(m - 1),
-                 (const S*)h->vec_a(m - 2), (S*)h->vec_a(m), h->xi.as<double>(), h->nb, h->qp,
-                 kx.chunk, 2.0, -1.0, 0.0, c[m], 0, kx.nt));
+    {
+      LoopRecorder rec(h);
+      // alpha_1 = Hs R, xi = c0 R + c1 alpha_1
+      CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(0),
+                 (const S*)nullptr, (S*)h->vec_a(1), h->xi.as<double>(), h->nb, h->qp, kx.chunk,
+                 1.0, 0.0, c[0], c[1], 1, kx.nt));
+      for (int m = 2; m < M; m++) {
+        // alpha_m = 2 Hs alpha_{m-1} - alpha_{m-2}; xi += c_m alpha_m
+        CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(m - 1),
+                   (const S*)h->vec_a(m - 2), (S*)h->vec_a(m), h->xi.as<double>(), h->nb, h->qp,
+                   kx.chunk, 2.0, -1.0, 0.0, c[m], 0, kx.nt));
+      }
+      CHK(rec.run());
     }
     CHK(timing_end(h, &h->ms_stoch, &h->n_stoch, M - 1));
     CHK(zero_doubles(h, h->dacc.p, dcount));
@@ -1032,10 +1209,11 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
   CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, (const S*)h->vec_a(0),
              (const S*)h->vec_a(0), dacc, h->nb, h->qp, ko.chunk, diag));
   int launches = 0;
+  if (M > 2)
+    for (int k = 0; k < 3; k++) CHK(h->Bv[k].ensure(h->vbytes()));
   CHK(timing_begin(h));
+  LoopRecorder rec(h);
   if (M > 2) {
-    const size_t vb = h->vbytes();
-    for (int k = 0; k < 3; k++) CHK(h->Bv[k].ensure(vb));
     const double sb = 2 * c[M - 1];  // beta_{M/2} = 2 c_{M-1} alpha_{M/2-1}
     const bool stored = h->a_stored && !ka.mfma;
     for (int m = M / 2 - 1; m >= 1; m--) {
@@ -1076,6 +1254,7 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
                (const S*)(stored ? h->vec_a(0) : h->A[0].p), (const S*)h->Bv[1].p, dacc, h->nb,
                h->qp, ko.chunk, 1.0));
   }
+  CHK(rec.run());
   CHK(timing_end(h, &h->ms_ad, &h->n_ad, launches));
   // symmetrize + scale(1/mag)
   const size_t dcnt = (size_t)h->nnzbD * h->b * h->b;
@@ -1882,6 +2061,13 @@ int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep) {
   return 0;
 }
 
+int fkpm_get_graph_stats(fkpm_handle* h, long long* launches, long long* captures) {
+  if (!h) return fail(FKPM_E_ARG, "null handle");
+  if (launches) *launches = h->n_graph_launches;
+  if (captures) *captures = h->n_graph_captures;
+  return 0;
+}
+
 int fkpm_get_state(fkpm_handle* h, int* r_compact, int* tuned, int* moments_M) {
   if (!h) return fail(FKPM_E_ARG, "null handle");
   if (r_compact) *r_compact = h->r_compact ? 1 : 0;
@@ -1939,6 +2125,10 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
             h->kc[k].sweep = value;
             derive_grid(h->nb, h->kc[k]);
           }
+      return 0;
+    case FKPM_OPT_GRAPH:
+      if (value < -1 || value > 1) return fail(FKPM_E_ARG, "bad graph switch %d", value);
+      h->graph_mode = value;
       return 0;
     case FKPM_OPT_SLICE:
       if (value < -1 || value > 1) return fail(FKPM_E_ARG, "bad slice switch %d", value);

@@ -316,6 +316,16 @@ class HipEngine:
         back to back, -1 the configuration's own."""
         call("fkpm_set_option", self._h, _capi.OPT_SLICE, int(mode))
 
+    def set_graph(self, mode: int) -> None:
+        """Step loops as hipGraphs (FKPM_OPT_GRAPH): 1 always, 0 never, -1
+        automatic (short steps)."""
+        call("fkpm_set_option", self._h, _capi.OPT_GRAPH, int(mode))
+
+    def graph_stats(self) -> dict:
+        a, b = C.c_longlong(), C.c_longlong()
+        call("fkpm_get_graph_stats", self._h, C.byref(a), C.byref(b))
+        return {"graph_launches": a.value, "graph_captures": b.value}
+
     def set_alpha_table_mb(self, mib: int) -> None:
         """Most MiB the table of forward vectors may take on this handle
         (FKPM_OPT_ALPHA_TABLE_MB; -1: a quarter of the device, at most half

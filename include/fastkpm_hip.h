@@ -179,6 +179,14 @@ int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
                                    its L2; 0: a row's column passes back to back; -1
                                    (default): the configuration's own (the tuner's choice,
                                    else 0) */
+#define FKPM_OPT_GRAPH 8        /* value 1: the launches of every step loop (moments, the
+                                   stoch_matrix recurrences, the autodiff backward steps)
+                                   replay as one hipGraph, captured the first time the same
+                                   launches (kernels, grids, argument values) occur and kept
+                                   for the handle (the 16 most recent); 0: direct launches;
+                                   -1 (default): graphs when the recursion vectors hold at most
+                                   2^22 scalars (steps short enough for launches to set the
+                                   time) */
 #define FKPM_OPT_ALPHA_TABLE_MB 7 /* value >= 0: the most MiB the table of forward vectors
                                    (FKPM_OPT_STORE_ALPHA) may hold on this handle; -1
                                    (default): a quarter of the device's memory and at most
@@ -230,6 +238,9 @@ int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep);
 /* *stored = 1 when the last fkpm_moments kept its forward vectors for the
  * next fkpm_autodiff_matrix (FKPM_OPT_STORE_ALPHA). */
 int fkpm_get_alpha_table(fkpm_handle* h, int* stored);
+
+/* Step loops replayed as a hipGraph and graphs captured so far (FKPM_OPT_GRAPH). */
+int fkpm_get_graph_stats(fkpm_handle* h, long long* launches, long long* captures);
 
 /* *method = the FKPM_STOCH_* method the last fkpm_stoch_matrix ran (-1 if
  * none yet). */

@@ -216,3 +216,50 @@ def test_set_H_with_longer_rows_picks_kernels_again(oracle):
         D2 = H.like()
         eng.stoch_matrix(cF, D2)
         assert rel_err(D2.val, orc.stoch_matrix(cF, D2)) < FP64_TOL, deg(H)
+
+
+@pytest.mark.parametrize("method,store", [("auto", 2), ("clenshaw", 0), ("forward", 0)])
+def test_step_loops_as_graphs_equal_direct_launches(oracle, method, store):
+    """FKPM_OPT_GRAPH: the C1 solve (moments, autodiff_matrix, stoch_matrix)
+    with every step loop replayed as a cached hipGraph is bitwise the solve
+    with direct launches, on every replay, with other eager work (a different
+    M, so other graphs and buffers) between the replays; the first solve is
+    also checked against the oracle."""
+    from fastkpm_amd import EnergyScale, HipEngine, host
+    H, groups, cfg = _c1(np.complex128)
+    M, (lo, hi), kT, mu = cfg["M"], cfg["es"], cfg["kT"], cfg["mu"]
+    cE, cF = _coeffs(M, lo, hi, kT, mu)
+    c2E, _ = _coeffs(64, lo, hi, kT, mu)
+
+    def solve(eng, m=M, ce=cE, cf=cF):
+        mu_ = eng.moments(m)
+        D1, D2 = H.like(), H.like()
+        eng.autodiff_matrix(ce, D1)
+        eng.stoch_matrix(cf, D2)
+        return mu_, D1.val, D2.val
+
+    engines = {}
+    for g in (0, 1):
+        eng = HipEngine("cx_double", "fp64")
+        eng.set_graph(g)
+        eng.set_store_alpha(store)
+        eng.set_stoch_method(method)
+        eng.set_R_correlated(groups, host.RNG(2024))
+        eng.set_H(H, EnergyScale(lo, hi))
+        engines[g] = eng
+    ref = solve(engines[0])
+    orc = oracle.Engine()
+    orc.set_H(H, lo, hi)
+    orc.set_R(engines[0].R)
+    assert rel_err(ref[0], orc.moments(M)) < FP64_TOL
+    for rep in range(4):
+        out = solve(engines[1])
+        for a, b in zip(out, ref):
+            assert np.array_equal(a, b), rep
+        solve(engines[1], 64, c2E, c2E)  # eager work on other graphs in between
+        engines[0].synchronize()
+    st = engines[1].graph_stats()
+    # three step loops per solve, two solve shapes: each loop captured once,
+    # then replayed (4 repetitions x 2 solves x 3 loops launches)
+    assert st["graph_captures"] == 6 and st["graph_launches"] == 24
+    assert engines[0].graph_stats()["graph_launches"] == 0
