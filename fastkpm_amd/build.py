@@ -22,7 +22,7 @@ BUILD = PKG / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["kpm_capi.hip", "kpm_inst_dc.hip", "kpm_inst_dr.hip", "kpm_inst_fc.hip",
-               "kpm_inst_fr.hip", "kpm_batch.hip"]
+               "kpm_inst_fr.hip", "kpm_batch.hip", "kpm_noalloc.hip"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              # packed FP32 instructions (v_pk_*_f32) stay off: every kernel is built
              # from plain scalar FP32 code

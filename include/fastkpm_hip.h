@@ -152,6 +152,27 @@ int fkpm_get_config(fkpm_handle* h, int* lanes, int* kq, int* s_padded, int* n_w
  * Takes effect at the next call; discards the moments state. */
 int fkpm_set_geometry(fkpm_handle* h, int lanes, int kq, int max_wg);
 
+/* Chebyshev moments with no device allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * work of fkpm_set_H + fkpm_set_R + fkpm_moments in one call, on inputs that
+ * are already in device memory: H as BSR (nb block rows of b x b
+ * complex-double blocks, column-major blocks, rows sorted, every diagonal
+ * block present; row_ptr, col_idx, val all device pointers; the pattern is not
+ * checked), scaled as (H - avg) / mag; R (n x s complex double, column-major,
+ * n = nb * b, device); mu: M doubles in device memory.  The work is queued on
+ * `stream` (a hipStream_t; NULL: the null stream) and the call returns without
+ * waiting; prec = FKPM_PREC_FP64 or FKPM_PREC_FP32.  Working memory is a
+ * static device buffer of FKPM_NOALLOC_SCRATCH_MB MiB per device (the scaled
+ * H, the packed R, three recursion vectors and the moment partials); a
+ * problem that needs more returns FKPM_E_ARG.  Every call on one device shares
+ * that buffer: calls that overlap on one device (two streams or two threads)
+ * race.  The untuned vector-unit kernels run (no matrix cores, no tuner). */
+#define FKPM_NOALLOC_SCRATCH_MB 64
+int fkpm_moments_noalloc(int device, int prec, int nb, int b, int nnzb, const int* row_ptr,
+                         const int* col_idx, const void* val, double avg, double mag, int n,
+                         int s, const void* R_colmajor, int M, double* mu, void* stream);
+
 /* Options (fkpm_set_option). */
 #define FKPM_OPT_STOCH_METHOD 1 /* value: FKPM_STOCH_AUTO (default), FKPM_STOCH_CLENSHAW or
                                    FKPM_STOCH_FORWARD */
