@@ -101,20 +101,24 @@ def algorithmic_bytes(H, N: int, S: int, v: int, vacc: int = 16, table_K: int | 
     }
 
 
-def measured_traffic(config: str, prec: str, kernel: str):
+def measured_traffic(config: str, prec: str, kernel: str, kc: dict, b: int):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE measurement of this workload
-    (profiles/traffic_<config>_<prec>.json), or None."""
+    (profiles/traffic_<config>_<prec>.json): the entry of the same kernel,
+    storage type, block size and geometry at the same grid, or None."""
     path = ROOT / "profiles" / f"traffic_{config.lower()}_{prec}.json"
     if not path.exists():
         return None
     data = json.loads(path.read_text())
+    st = "double" if prec == "fp64" else "float"
+    stem = f"::{kernel.split(' ')[0]}<{st}, true, {b}, {kc['lanes']}, {kc['kq']}"
     for k in data.get("kernels", []):
-        if "::" + kernel.split(" ")[0] + "<" in k["kernel"]:
+        if stem in k["kernel"] and k["grid_threads"] == 256 * kc["n_wg"]:
             out = {"traffic_bytes_per_launch": k["traffic_bytes_per_launch"],
                    "source": f"{path.name} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"}
-            if "request_bytes_per_launch" in k:
-                out["request_bytes_per_launch"] = k["request_bytes_per_launch"]
+            for key in ("request_bytes_per_launch", "read_request_bytes_per_launch"):
+                if k.get(key):
+                    out[key] = k[key]
             return out
     return None
 
@@ -565,13 +569,16 @@ def main(argv=None) -> int:
                         "frac": round(kern[dom]["achieved_TFLOPs"] / peak_tf, 4)}
     roof["copy_GBs"] = round(copy_gbs, 1)
     roof["frac_of_copy"] = round(kern[dom]["achieved_GBs"] / copy_gbs, 4)
-    tr = measured_traffic(args.config, args.precision, names[dom])
+    tr = None if kc.get("mfma") else measured_traffic(args.config, args.precision, names[dom],
+                                                        kc, H.b_len)
     if tr is not None:
         roof["traffic"] = tr["traffic_bytes_per_launch"]
         roof["traffic_source"] = tr["source"]
         if "request_bytes_per_launch" in tr:
             # the same bytes from the EA request counters by size (32/64/128 B)
             roof["traffic_by_request_size"] = tr["request_bytes_per_launch"]
+        if "read_request_bytes_per_launch" in tr:
+            roof["read_bytes_by_request_size"] = tr["read_request_bytes_per_launch"]
     times_ms = np.array(r["times"]) * 1e3
     out = {
         "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,
