@@ -319,6 +319,29 @@ def cpu_baseline(H, groups, cfg, es, seed: int, m_sample: int):
                        f"{os.cpu_count()} cores")}
 
 
+def single_engine_rate(specs, built, prec: str) -> dict:
+    """N*M*S per second of the requests solved one by one on HipEngine
+    (moments + autodiff_matrix, D to the host) with H and R set before each
+    timed solve and the untuned default kernels, as the batched engine runs
+    them."""
+    t_tot, upd = 0.0, 0
+    for sp, (H, es, c, (groups, seeds)) in zip(specs, built):
+        eng = HipEngine("cx_double", prec)
+        eng.set_R_correlated(groups, host.RNG(sp.seed))
+        eng.set_H(H, es)
+        D = H.like()
+        eng.moments(sp.M)  # first use of the shape's buffers
+        eng.autodiff_matrix(c, D)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.moments(sp.M)
+        eng.autodiff_matrix(c, D)
+        t_tot += time.perf_counter() - t0
+        upd += sp.N * sp.M * sp.S
+        eng.close()
+    return {"updates_per_s": upd / t_tot, "solve_s_total": t_tot}
+
+
 def serving_leg(n_req: int, n_max: int, m_max: int, rate: float, seed: int, hbm_gb: int,
                 precs=("fp64", "fp32"), graph: bool = True) -> dict:
     """The mixed Poisson request stream through the batched engine."""
@@ -348,6 +371,12 @@ def serving_leg(n_req: int, n_max: int, m_max: int, rate: float, seed: int, hbm_
         summ.update(eng.stats())
         out[prec] = summ
         eng.close()
+        # the same work on the single-request engine, one request after
+        # another (moments + autodiff_matrix with H and R already set): the
+        # rate the batched engine is held to
+        summ["single_engine"] = single_engine_rate(specs, built, prec)
+        summ["saturated_vs_single_engine"] = (summ["saturated"]["updates_per_s"]
+                                              / summ["single_engine"]["updates_per_s"])
     if "fp64" in out and "fp32" in out:
         e64 = np.array(out["fp64"].pop("energies"))
         e32 = np.array(out["fp32"].pop("energies"))

@@ -1776,6 +1776,11 @@ int fkpm_dev_open(int device, int dtype, int prec, fkpm_handle** out) {
   // FKPM_AUTOTUNE=1: tune at the first use of a shape (FKPM_OPT_AUTOTUNE for
   // callers that cannot reach fkpm_set_option, such as the C++ API)
   if (const char* e = getenv("FKPM_AUTOTUNE")) h->autotune_on_use = e[0] != '\0' && e[0] != '0';
+  // FKPM_GRAPH=0/1/-1: the FKPM_OPT_GRAPH default for handles opened afterwards
+  if (const char* e = getenv("FKPM_GRAPH")) {
+    const int v = atoi(e);
+    if (v >= -1 && v <= 1) h->graph_mode = v;
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
     delete h;

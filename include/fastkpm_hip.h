@@ -207,7 +207,8 @@ int fkpm_moments_noalloc(int device, int prec, int nb, int b, int nnzb, const in
                                    for the handle (the 16 most recent); 0: direct launches;
                                    -1 (default): graphs when the recursion vectors hold at most
                                    2^22 scalars (steps short enough for launches to set the
-                                   time) */
+                                   time); the environment variable FKPM_GRAPH (-1, 0 or 1) at
+                                   fkpm_dev_open sets the handle's default */
 #define FKPM_OPT_ALPHA_TABLE_MB 7 /* value >= 0: the most MiB the table of forward vectors
                                    (FKPM_OPT_STORE_ALPHA) may hold on this handle; -1
                                    (default): a quarter of the device's memory and at most
