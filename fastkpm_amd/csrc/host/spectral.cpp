@@ -7,11 +7,174 @@
 #include <algorithm>
 #include <cassert>
 #include <cmath>
+#include <complex>
 #include <stdexcept>
+#include <vector>
 
 #include "fastkpm.h"
 
 namespace fkpm {
+
+namespace {
+
+using cd = std::complex<double>;
+
+// In-place DFT of a power-of-two length n, a_k <- sum_j a_j exp(sign 2 pi i j k / n),
+// iterative radix 2; the bit-reversal permutation and the twiddle factors
+// (each evaluated directly, not by recurrence) are built once per length.
+class Pow2Fft {
+ public:
+  explicit Pow2Fft(size_t n) : n_(n), rev_(n), tw_(n / 2) {
+    for (size_t i = 1, j = 0; i < n; i++) {
+      size_t bit = n >> 1;
+      for (; j & bit; bit >>= 1) j ^= bit;
+      j ^= bit;
+      rev_[i] = j;
+    }
+    for (size_t k = 0; k < n / 2; k++) tw_[k] = std::polar(1.0, -2.0 * Pi * (double)k / (double)n);
+  }
+  size_t size() const { return n_; }
+  void run(std::vector<cd>& a, int sign) const {
+    for (size_t i = 1; i < n_; i++)
+      if (i < rev_[i]) std::swap(a[i], a[rev_[i]]);
+    for (size_t len = 2; len <= n_; len <<= 1) {
+      const size_t half = len / 2, step = n_ / len;
+      for (size_t i = 0; i < n_; i += len)
+        for (size_t k = 0; k < half; k++) {
+          const cd w = sign < 0 ? tw_[k * step] : std::conj(tw_[k * step]);
+          const cd u = a[i + k], v = a[i + k + half] * w;
+          a[i + k] = u + v;
+          a[i + k + half] = u - v;
+        }
+    }
+  }
+
+ private:
+  size_t n_;
+  std::vector<size_t> rev_;
+  std::vector<cd> tw_;
+};
+
+// DFT of any length n > 0, X_k = sum_j x_j exp(sign 2 pi i j k / n): the
+// radix-2 FFT itself for a power of two, otherwise Bluestein's chirp-z
+// identity j k = (j^2 + k^2 - (k - j)^2) / 2, which makes it a cyclic
+// convolution of length L >= 2n - 1 (a power of two) with the chirp, whose
+// transform is kept in the plan (two FFTs of length L per call).
+class Dft {
+ public:
+  Dft(size_t n, int sign) : n_(n), sign_(sign), fft_(pow2_for(n)) {
+    if (pow2(n)) return;
+    const size_t L = fft_.size();
+    c_.resize(n);  // exp(sign pi i j^2 / n), j^2 reduced mod 2n
+    for (size_t j = 0; j < n; j++) {
+      const unsigned long long r = ((unsigned long long)j * j) % (2ull * n);
+      c_[j] = std::polar(1.0, sign * Pi * (double)r / (double)n);
+    }
+    bhat_.assign(L, cd(0.0, 0.0));
+    bhat_[0] = std::conj(c_[0]);
+    for (size_t m = 1; m < n; m++) bhat_[m] = bhat_[L - m] = std::conj(c_[m]);
+    fft_.run(bhat_, -1);
+    for (cd& v : bhat_) v /= (double)L;  // the inverse transform's 1/L, folded in once
+  }
+  // x holds n values on entry and the transform on return.
+  void run(std::vector<cd>& x) const {
+    if (c_.empty()) {
+      fft_.run(x, sign_);
+      return;
+    }
+    const size_t L = fft_.size();
+    std::vector<cd>& a = work_;
+    a.assign(L, cd(0.0, 0.0));
+    for (size_t j = 0; j < n_; j++) a[j] = x[j] * c_[j];
+    fft_.run(a, -1);
+    for (size_t k = 0; k < L; k++) a[k] *= bhat_[k];
+    fft_.run(a, +1);
+    for (size_t k = 0; k < n_; k++) x[k] = c_[k] * a[k];
+  }
+
+ private:
+  static bool pow2(size_t n) { return (n & (n - 1)) == 0; }
+  static size_t pow2_for(size_t n) {
+    if (pow2(n)) return n;
+    size_t L = 1;
+    while (L < 2 * n - 1) L <<= 1;
+    return L;
+  }
+  size_t n_;
+  int sign_;
+  Pow2Fft fft_;
+  std::vector<cd> c_, bhat_;
+  mutable std::vector<cd> work_;
+};
+
+// DCT-II and DST-II of n real values (the FFTW REDFT10 and RODFT10 conventions):
+//   yc_k = 2 sum_j x_j cos(pi (j + 1/2) k / n),
+//   ys_k = 2 sum_j x_j sin(pi (j + 1/2) (k + 1) / n),   k = 0 .. n-1,
+// both from one DFT X of length 2n of x padded with zeros:
+//   yc_k = 2 Re(exp(-i pi k / 2n) X_k),  ys_k = -2 Im(exp(-i pi (k+1) / 2n) X_{k+1}).
+class Dct2Dst2 {
+ public:
+  explicit Dct2Dst2(size_t n) : n_(n), dft_(2 * n, -1), ph_(n + 1) {
+    for (size_t k = 0; k <= n; k++) ph_[k] = std::polar(1.0, -Pi * (double)k / (2.0 * n));
+  }
+  void run(const std::vector<double>& x, std::vector<double>* yc, std::vector<double>* ys) const {
+    std::vector<cd>& z = work_;
+    z.assign(2 * n_, cd(0.0, 0.0));
+    for (size_t j = 0; j < n_; j++) z[j] = x[j];
+    dft_.run(z);
+    if (yc) {
+      yc->resize(n_);
+      for (size_t k = 0; k < n_; k++) (*yc)[k] = 2.0 * (ph_[k] * z[k]).real();
+    }
+    if (ys) {
+      ys->resize(n_);
+      for (size_t k = 0; k < n_; k++) (*ys)[k] = -2.0 * (ph_[k + 1] * z[k + 1]).imag();
+    }
+  }
+
+ private:
+  size_t n_;
+  Dft dft_;
+  std::vector<cd> ph_;
+  mutable std::vector<cd> work_;
+};
+
+// Chebyshev synthesis at the Mq Chebyshev nodes (a DCT-III) of M <= Mq complex
+// coefficients:
+//   g_i = sum_{m < M} a_m T_m(x_i) = sum_m a_m cos(pi m (i + 1/2) / Mq),  i < Mq.
+// For real r, sum_m r_m cos(pi m (2i + 1) / 2Mq) = Re sum_m (r_m e^{i pi m / 2Mq}) e^{2 pi i m i / 2Mq},
+// so the real and imaginary parts of a take one DFT of length 2Mq each.
+class ChebSynthesis {
+ public:
+  ChebSynthesis(size_t M, size_t Mq) : M_(M), Mq_(Mq), dft_(2 * Mq, +1), ph_(M) {
+    for (size_t m = 0; m < M; m++) ph_[m] = std::polar(1.0, Pi * (double)m / (2.0 * Mq));
+  }
+  void run(const cd* a, std::vector<cd>& out) const {
+    out.assign(Mq_, cd(0.0, 0.0));
+    std::vector<cd>& z = work_;
+    for (int part = 0; part < 2; part++) {
+      z.assign(2 * Mq_, cd(0.0, 0.0));
+      bool any = false;
+      for (size_t m = 0; m < M_; m++) {
+        const double r = part == 0 ? a[m].real() : a[m].imag();
+        any |= r != 0.0;
+        z[m] = r * ph_[m];
+      }
+      if (!any) continue;
+      dft_.run(z);
+      for (size_t i = 0; i < Mq_; i++)
+        out[i] += part == 0 ? cd(z[i].real(), 0.0) : cd(0.0, z[i].real());
+    }
+  }
+
+ private:
+  size_t M_, Mq_;
+  Dft dft_;
+  std::vector<cd> ph_;
+  mutable std::vector<cd> work_;
+};
+
+}  // namespace
 
 // src/fastkpm.cpp:13-20
 Vec<double> jackson_kernel(int M) {
@@ -75,10 +238,18 @@ Vec<double> moment_transform(Vec<double> const& moments, int Mq) {
   return gamma;
 }
 
-// src/fastkpm.cpp:80-233, the direct-sum branch (no FFTW in this build):
-// static (omega = 0) Kubo-Bastin coefficients from the Chebyshev nodes with
-// 1 - x^2 >= 1e-4, or the optical ones from the nodes i >= i_start; both
-// times (2 - delta_m1,0)(2 - delta_m2,0) g_m1 g_m2 * 2 pi / (Mq mag^2).
+// src/fastkpm.cpp:80-233, the branch the reference takes with FFTW, on this
+// library's own transforms (Dct2Dst2 above):
+// - static (omega = 0): with f_i the Fermi density at node x_i, t_i = 1 - x_i^2
+//   and nodes t_i < 1e-4 left out, DCT-II and DST-II of x1_i = f_i x_i / t_i^1.5
+//   and x2_i = f_i / t_i give every C_{m1,m2} from the sums and differences of
+//   m1, m2 (Mq >= 2M keeps m1 + m2 inside the transforms), times
+//   pi (2 - delta_m1,0)(2 - delta_m2,0) g_m1 g_m2 / (2 Mq mag^2);
+// - optical: for each m1 one DCT-II over the nodes i >= i_start of
+//   T_m1(x_i + omega) (f(x_i) - f(x_i + omega)) / (2 omega sqrt(1 - (x_i + omega)^2)),
+//   times 2 pi (2 - delta_m1,0)(2 - delta_m2,0) g_m1 g_m2 / (Mq mag^2).
+// O(Mq log Mq + M^2) and O(M Mq log Mq) instead of the direct node sums'
+// O(M^2 Mq); the unit is e^2/h (the factor 2 pi).
 Vec<Vec<cx_double>> electrical_conductivity_coefficients(int M, int Mq, double kT, double mu,
                                                          double omega, EnergyScale es,
                                                          Vec<double> const& kernel) {
@@ -89,45 +260,61 @@ Vec<Vec<cx_double>> electrical_conductivity_coefficients(int M, int Mq, double k
   const double ws = omega / es.mag();
   Vec<Vec<cx_double>> ret(M, Vec<cx_double>(M, cx_double(0.0, 0.0)));
   if (ws >= 2.0) return ret;
-  Vec<double> Ti(M), Tj(M);
+  const double mag2 = es.mag() * es.mag();
   if (ws < 1.0e-10) {
+    std::vector<double> x1(Mq, 0.0), x2(Mq, 0.0);
     for (int i = 0; i < Mq; i++) {
       const double x_i = std::cos(Pi * (i + 0.5) / Mq);
-      if (1.0 - x_i * x_i < cutoff) continue;
-      chebyshev_fill_array(x_i, Ti);     // T_m(x) = cos(m t)
-      chebyshev_fill_array(x_i, Tj, 2);  // U_m(x)
-      const double s1 = std::sqrt(1.0 - x_i * x_i);
-      for (int m = M - 1; m > 0; m--) Tj[m] = Tj[m - 1] * s1;  // sin(m t)
-      Tj[0] = 0.0;
-      const double f_i = fermi_density(es.unscale(x_i), kT, mu) / std::pow(1.0 - x_i * x_i, 1.5);
-      for (int m1 = 0; m1 < M; m1++)
-        for (int m2 = 0; m2 < M; m2++)
-          ret[m1][m2] += (Ti[m1] * cx_double(Ti[m2], -Tj[m2]) * cx_double(x_i, m2 * s1) +
-                          Ti[m2] * cx_double(Ti[m1], Tj[m1]) * cx_double(x_i, -m1 * s1)) *
-                         f_i;
+      const double t = 1.0 - x_i * x_i;
+      if (t < cutoff) continue;
+      const double f_i = fermi_density(es.unscale(x_i), kT, mu);
+      x1[i] = f_i * x_i / std::pow(t, 1.5);
+      x2[i] = f_i / t;
     }
-  } else {
-    const int i_start = (int)std::ceil(std::acos(1.0 - ws) / Pi * Mq - 0.5);
-    if (Mq - i_start < 20)
-      throw std::invalid_argument("conductivity: fewer than 20 nodes below 1 - omega");
-    for (int i = i_start; i < Mq; i++) {
-      const double x_i = std::cos(Pi * (i + 0.5) / Mq);
-      const double xw = x_i + ws;
-      if (x_i < -1.0 + cutoff || xw > 1.0 - cutoff) continue;
-      chebyshev_fill_array(xw, Ti);   // T_m(x + omega)
-      chebyshev_fill_array(x_i, Tj);  // T_m(x)
-      const double s2 = std::sqrt(1.0 - xw * xw);
-      const double f_i =
-          (fermi_density(es.unscale(x_i), kT, mu) - fermi_density(es.unscale(x_i) + omega, kT, mu)) /
-          (ws * s2);
-      for (int m1 = 0; m1 < M; m1++)
-        for (int m2 = 0; m2 < M; m2++) ret[m1][m2] += Ti[m1] * Tj[m2] * f_i;
+    std::vector<double> yc1, yc2, ys1, ys2;
+    const Dct2Dst2 tr(Mq);
+    tr.run(x1, &yc1, &ys1);
+    tr.run(x2, &yc2, &ys2);
+    auto ce = [](const std::vector<double>& y, int k) { return y[k >= 0 ? k : -k]; };
+    auto se = [](const std::vector<double>& y, int k) {
+      return k > 0 ? y[k - 1] : (k < 0 ? -y[-k - 1] : 0.0);
+    };
+    for (int m1 = 0; m1 < M; m1++) {
+      const double t1 = Pi * (m1 == 0 ? 1.0 : 2.0) * kernel[m1] / (2.0 * Mq * mag2);
+      for (int m2 = 0; m2 < M; m2++) {
+        const double pref = t1 * (m2 == 0 ? 1.0 : 2.0) * kernel[m2];
+        const int ms = m1 + m2, md = m1 - m2;
+        const double y_re = 2.0 * yc1[ms] + 2.0 * ce(yc1, md) + (ms > 0 ? ms * ys2[ms - 1] : 0.0) +
+                            md * se(ys2, md);
+        const double y_im = 2.0 * se(ys1, md) - md * (yc2[ms] + ce(yc2, md));
+        ret[m1][m2] = cx_double(pref * y_re, pref * y_im);
+      }
     }
+    return ret;
   }
+  const int i_start = (int)std::ceil(std::acos(1.0 - ws) / Pi * Mq - 0.5);
+  if (Mq - i_start < 20)
+    throw std::invalid_argument("conductivity: fewer than 20 nodes below 1 - omega");
+  std::vector<double> w(Mq, 0.0), acw(Mq, 0.0);  // node weights and arccos(x_i + omega)
+  std::vector<char> use(Mq, 0);
+  for (int i = i_start; i < Mq; i++) {
+    const double x_i = std::cos(Pi * (i + 0.5) / Mq);
+    const double xw = x_i + ws;
+    if (x_i < -1.0 + cutoff || xw > 1.0 - cutoff) continue;
+    use[i] = 1;
+    acw[i] = std::acos(xw);
+    w[i] = 0.5 * (fermi_density(es.unscale(x_i), kT, mu) -
+                  fermi_density(es.unscale(x_i) + omega, kT, mu)) /
+           (ws * std::sqrt(1.0 - xw * xw));
+  }
+  std::vector<double> X(Mq), Y;
+  const Dct2Dst2 tr(Mq);
   for (int m1 = 0; m1 < M; m1++) {
-    // the factor 2 pi changes the unit from e^2/hbar to e^2/h
-    const double t1 = 2.0 * Pi * (m1 == 0 ? 1.0 : 2.0) * kernel[m1] / (Mq * es.mag() * es.mag());
-    for (int m2 = 0; m2 < M; m2++) ret[m1][m2] *= t1 * (m2 == 0 ? 1.0 : 2.0) * kernel[m2];
+    for (int i = 0; i < Mq; i++) X[i] = use[i] ? std::cos(m1 * acw[i]) * w[i] : 0.0;
+    tr.run(X, &Y, nullptr);
+    const double t1 = 2.0 * Pi * (m1 == 0 ? 1.0 : 2.0) * kernel[m1] / (Mq * mag2);
+    for (int m2 = 0; m2 < M; m2++)
+      ret[m1][m2] = cx_double(Y[m2] * t1 * (m2 == 0 ? 1.0 : 2.0) * kernel[m2], 0.0);
   }
   return ret;
 }
@@ -143,28 +330,29 @@ cx_double moment_product(Vec<Vec<cx_double>> const& c, Vec<Vec<cx_double>> const
 }
 
 // src/fastkpm.cpp:258-289: gamma[i][j] = sum T_m1(x_i) T_m2(x_j) mup[m1][m2],
-// evaluated as two matrix products over the node table T[i][m] = T_m(x_i).
+// evaluated as Chebyshev syntheses (DCT-III, ChebSynthesis above) along m2
+// for every m1 and then along m1 for every node j: O(M Mq log Mq + Mq^2 log Mq)
+// instead of the reference's O(Mq^2 M^2) quadruple loop.
 Vec<Vec<cx_double>> moment_transform(Vec<Vec<cx_double>> const& moments, int Mq,
                                      Vec<double> const& kernel) {
   const int M = (int)moments.size();
   if ((int)kernel.size() != M) throw std::invalid_argument("moment_transform: kernel size != M");
-  Vec<Vec<cx_double>> mup(M, Vec<cx_double>(M));
-  for (int a = 0; a < M; a++)
+  if (Mq < M) throw std::invalid_argument("moment_transform: Mq must be >= M");
+  const ChebSynthesis syn(M, Mq);
+  std::vector<cd> row(M);
+  Vec<std::vector<cd>> G(M);  // G[m1][j] = sum_m2 mup[m1][m2] T_m2(x_j)
+  for (int a = 0; a < M; a++) {
     for (int b = 0; b < M; b++)
-      mup[a][b] = (a == 0 ? 1.0 : 2.0) * (b == 0 ? 1.0 : 2.0) * kernel[a] * kernel[b] * moments[a][b];
-  Vec<Vec<double>> T(Mq, Vec<double>(M));
-  for (int i = 0; i < Mq; i++) chebyshev_fill_array(std::cos(Pi * (i + 0.5) / Mq), T[i]);
-  Vec<Vec<cx_double>> G(Mq, Vec<cx_double>(M, cx_double(0.0, 0.0)));  // G = T mup
-  for (int i = 0; i < Mq; i++)
-    for (int a = 0; a < M; a++)
-      for (int b = 0; b < M; b++) G[i][b] += T[i][a] * mup[a][b];
-  Vec<Vec<cx_double>> gamma(Mq, Vec<cx_double>(Mq, cx_double(0.0, 0.0)));  // gamma = G T^t
-  for (int i = 0; i < Mq; i++)
-    for (int j = 0; j < Mq; j++) {
-      cx_double g(0.0, 0.0);
-      for (int b = 0; b < M; b++) g += G[i][b] * T[j][b];
-      gamma[i][j] = g;
-    }
+      row[b] = (a == 0 ? 1.0 : 2.0) * (b == 0 ? 1.0 : 2.0) * kernel[a] * kernel[b] * moments[a][b];
+    syn.run(row.data(), G[a]);
+  }
+  Vec<Vec<cx_double>> gamma(Mq, Vec<cx_double>(Mq, cx_double(0.0, 0.0)));
+  std::vector<cd> col(M), g;
+  for (int j = 0; j < Mq; j++) {
+    for (int a = 0; a < M; a++) col[a] = G[a][j];
+    syn.run(col.data(), g);
+    for (int i = 0; i < Mq; i++) gamma[i][j] = g[i];
+  }
   return gamma;
 }
 

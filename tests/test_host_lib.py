@@ -92,8 +92,8 @@ def test_filling_to_mu_inverts_filling(host, oracle):
 
 def _dct_form_static(oracle, M, Mq, kT, mu, lo, hi, kernel):
     """Static conductivity coefficients from DCT-II / DST-II transforms of two
-    node functions (the FFTW branch of src/fastkpm.cpp:90-140): an algebraically
-    different route from the library's direct node sum."""
+    node functions (the FFTW branch of src/fastkpm.cpp:90-140), on scipy's
+    transforms."""
     from scipy.fft import dct, dst
     avg, mag = (hi + lo) / 2, (hi - lo) / 2
     x = np.cos(np.pi * (np.arange(Mq) + 0.5) / Mq)
@@ -175,6 +175,94 @@ def test_conductivity_edges(host):
         host.conductivity_coefficients(M, 32, 0.1, 0.0, -0.1, -1.0, 1.0)
     with pytest.raises(ValueError, match="20 nodes"):
         host.conductivity_coefficients(M, 32, 0.1, 0.0, 1.9, -1.0, 1.0)
+
+
+def _direct_static(oracle, M, Mq, kT, mu, lo, hi, kernel):
+    """Static coefficients as the node sum of the branch without FFTW
+    (src/fastkpm.cpp:144-174): sum over nodes of T_m1 (T_m2 - i sin m2 t)
+    (x + i m2 sqrt(1 - x^2)) + (m1 <-> m2, conjugated), weighted by
+    f(x) / (1 - x^2)^1.5."""
+    avg, mag = (hi + lo) / 2, (hi - lo) / 2
+    th = np.pi * (np.arange(Mq) + 0.5) / Mq
+    x, s1 = np.cos(th), np.sin(th)
+    keep = 1.0 - x * x >= 1e-4
+    th, x, s1 = th[keep], x[keep], s1[keep]
+    f = np.array([oracle.fermi_density(xi * mag + avg, kT, mu) for xi in x]) / s1 ** 3
+    m = np.arange(M)
+    T, S = np.cos(np.outer(th, m)), np.sin(np.outer(th, m))
+    Q1 = (T - 1j * S) * (x[:, None] + 1j * m[None, :] * s1[:, None])
+    Q2 = (T + 1j * S) * (x[:, None] - 1j * m[None, :] * s1[:, None])
+    out = (T * f[:, None]).T @ Q1 + (Q2 * f[:, None]).T @ T
+    w = np.where(m == 0, 1.0, 2.0) * kernel
+    return out * (2 * np.pi * np.outer(w, w) / (Mq * mag * mag))
+
+
+def _direct_optical(oracle, M, Mq, kT, mu, omega, lo, hi, kernel):
+    """Optical coefficients as the node sum of the branch without FFTW
+    (src/fastkpm.cpp:203-222): sum over nodes of T_m1(x + omega) T_m2(x)
+    (f(x) - f(x + omega)) / (omega sqrt(1 - (x + omega)^2))."""
+    avg, mag = (hi + lo) / 2, (hi - lo) / 2
+    ws = omega / mag
+    i0 = int(np.ceil(np.arccos(1.0 - ws) / np.pi * Mq - 0.5))
+    x = np.cos(np.pi * (np.arange(i0, Mq) + 0.5) / Mq)
+    xw = x + ws
+    keep = (x >= -1 + 1e-4) & (xw <= 1 - 1e-4)
+    x, xw = x[keep], xw[keep]
+    e = x * mag + avg
+    df = np.array([oracle.fermi_density(a, kT, mu) - oracle.fermi_density(a + omega, kT, mu)
+                   for a in e])
+    f = df / (ws * np.sqrt(1 - xw * xw))
+    m = np.arange(M)
+    T1, T2 = np.cos(np.outer(np.arccos(xw), m)), np.cos(np.outer(np.arccos(x), m))
+    w = np.where(m == 0, 1.0, 2.0) * kernel
+    return (T1 * f[:, None]).T @ T2 * (2 * np.pi * np.outer(w, w) / (Mq * mag * mag))
+
+
+@pytest.mark.parametrize("M,Mq", [(96, 300), (64, 256), (50, 101)])
+def test_conductivity_static_matches_node_sum(host, oracle, M, Mq):
+    """The transform route (power-of-two and chirp-z lengths) against the
+    direct node sum, a different algebraic route."""
+    lo, hi, kT, mu = -3.2, 2.6, 0.05, -0.4
+    g = host.jackson_kernel(M)
+    got = host.conductivity_coefficients(M, Mq, kT, mu, 0.0, lo, hi)
+    want = _direct_static(oracle, M, Mq, kT, mu, lo, hi, g)
+    assert np.abs(got - want).max() < 1e-11 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("M,Mq,omega", [(64, 300, 0.5), (40, 128, 1.3)])
+def test_conductivity_optical_matches_node_sum(host, oracle, M, Mq, omega):
+    lo, hi, kT, mu = -3.0, 3.0, 0.1, 0.2
+    g = host.jackson_kernel(M)
+    got = host.conductivity_coefficients(M, Mq, kT, mu, omega, lo, hi)
+    want = _direct_optical(oracle, M, Mq, kT, mu, omega, lo, hi, g)
+    assert np.abs(got - want).max() < 1e-11 * np.abs(want).max()
+
+
+def test_conductivity_large_M_is_fast_and_hermitian(host):
+    """M = 2000 with Mq = 4000 (the expansion orders of the large workloads)
+    finishes in seconds; the static coefficients satisfy
+    C_{m2,m1} = conj(C_{m1,m2}) (the integrand's two terms swap under
+    m1 <-> m2 with conjugation)."""
+    import time
+    M, Mq = 2000, 4000
+    t0 = time.perf_counter()
+    C = host.conductivity_coefficients(M, Mq, 0.05, 0.1, 0.0, -7.0, 7.0)
+    assert time.perf_counter() - t0 < 10.0
+    assert np.abs(C - C.conj().T).max() < 1e-10 * np.abs(C).max()
+
+
+def test_moment_transform2_matches_direct_sum(host):
+    """gamma[i][j] = sum T_m1(x_i) T_m2(x_j) (2 - delta)(2 - delta) g g mu[m1][m2]
+    (the quadruple loop of src/fastkpm.cpp:258-289), at a chirp-z length."""
+    rng = np.random.default_rng(9)
+    M, Mq = 17, 45
+    mu = rng.normal(size=(M, M)) + 1j * rng.normal(size=(M, M))
+    g = host.jackson_kernel(M)
+    w = np.where(np.arange(M) == 0, 1.0, 2.0) * g
+    T = np.cos(np.outer(np.arange(M), np.pi * (np.arange(Mq) + 0.5) / Mq))
+    want = T.T @ (np.outer(w, w) * mu) @ T
+    got = host.moment_transform2(mu, Mq)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
 
 
 def test_moment_transform2_is_separable(host):
