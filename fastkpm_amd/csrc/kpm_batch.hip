@@ -331,19 +331,21 @@ ClassDef mk_class(int W) {
                   &fkd::k_batch_bwd<S, B, L, KQ, MF>, &fkd::k_batch_aux<S, B, L, KQ, MF>};
 }
 
-// Complex 2x2 blocks: pass widths 16, 32, 64 packets; complex 8x8 blocks:
-// matrix cores (tiles of 16 columns) with the outer products at width 16
-// (FP64) or 8 (FP32) packets, general and row-resident (the last entry, taken
-// for requests whose rows all have at most MF_DM blocks).
+// Complex 2x2 blocks: pass widths 8, 16, 32, 64 packets (a packet holds one
+// FP64 or two FP32 complex columns; the narrowest class keeps a 16-column
+// FP32 request from being padded to 32); complex 8x8 blocks: matrix cores
+// (tiles of 16 columns) with the outer products at width 16 (FP64) or 8
+// (FP32) packets, general and row-resident (the last entry, taken for
+// requests whose rows all have at most MF_DM blocks).
 const std::vector<ClassDef>& classes(bool single) {
   static const std::vector<ClassDef> d = {
-      mk_class<double, 2, 8, 2, 0>(16), mk_class<double, 2, 16, 2, 0>(32),
-      mk_class<double, 2, 64, 1, 0>(64), mk_class<double, 8, 16, 1, 1>(16),
-      mk_class<double, 8, 16, 1, 2>(16)};
+      mk_class<double, 2, 4, 2, 0>(8),    mk_class<double, 2, 8, 2, 0>(16),
+      mk_class<double, 2, 16, 2, 0>(32),  mk_class<double, 2, 64, 1, 0>(64),
+      mk_class<double, 8, 16, 1, 1>(16),  mk_class<double, 8, 16, 1, 2>(16)};
   static const std::vector<ClassDef> f = {
-      mk_class<float, 2, 8, 2, 0>(16), mk_class<float, 2, 16, 2, 0>(32),
-      mk_class<float, 2, 64, 1, 0>(64), mk_class<float, 8, 8, 1, 1>(8),
-      mk_class<float, 8, 8, 1, 2>(8)};
+      mk_class<float, 2, 4, 2, 0>(8),    mk_class<float, 2, 8, 2, 0>(16),
+      mk_class<float, 2, 16, 2, 0>(32),  mk_class<float, 2, 64, 1, 0>(64),
+      mk_class<float, 8, 8, 1, 1>(8),    mk_class<float, 8, 8, 1, 2>(8)};
   return single ? f : d;
 }
 
