@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libfastkpm_hip.so"
 F32, F64, C32, C64 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 OPT_STOCH_METHOD, OPT_AUTOTUNE, OPT_MFMA, OPT_SWEEP, OPT_STORE_ALPHA = 1, 2, 3, 4, 5
-OPT_SLICE, OPT_ALPHA_TABLE_MB, OPT_GRAPH = 6, 7, 8
+OPT_SLICE, OPT_ALPHA_TABLE_MB, OPT_GRAPH, OPT_ROW_ORDER = 6, 7, 8, 9
 STOCH_CLENSHAW, STOCH_FORWARD, STOCH_TABLE, STOCH_AUTO = 0, 1, 2, 3
 KERNEL_MOMENTS, KERNEL_XI, KERNEL_AUTODIFF, KERNEL_OUTER = 0, 1, 2, 3
 KERNEL_MOMENTS_TABLE, KERNEL_AUTODIFF_TABLE = 4, 5

@@ -307,6 +307,9 @@ struct fkpm_handle {
   int store_alpha = 1;    // FKPM_OPT_STORE_ALPHA: 0 never, 1 auto, 2 whenever the table fits
   int sweep_force = -1;  // FKPM_OPT_SWEEP: row sweep of every kind (-1: per configuration)
   int slice_force = -1;  // FKPM_OPT_SLICE: column-pass loop order (-1: per configuration)
+  int order_force = -1;  // FKPM_OPT_ROW_ORDER: lattice row order (-1: per configuration)
+  bool has_order = false;  // rdesc holds a row order for the current pattern
+  DevBuf rdesc;            // {row, first block, end block} per sweep position
   long long table_cap_mb = -1;  // FKPM_OPT_ALPHA_TABLE_MB (-1: the default budget)
 
   // --- D ---
@@ -495,9 +498,35 @@ int set_kind(fkpm_handle* h, int kind, KCfg c) {
   if (h->sweep_force >= 0) c.sweep = h->sweep_force;
   if (h->slice_force >= 0 && !c.mfma)
     c.nt = (c.nt & ~fkd::NT_SLICE) | (h->slice_force ? fkd::NT_SLICE : 0);
+  if (h->order_force >= 0 && !c.mfma && kind != K_OUT)
+    c.nt = (c.nt & ~fkd::NT_ORDER) | (h->order_force ? fkd::NT_ORDER : 0);
+  // the order exists for lattice patterns only; its sweep is one range per XCD
+  if (!h->has_order || c.mfma || kind == K_OUT) c.nt &= ~fkd::NT_ORDER;
+  if (c.nt & fkd::NT_ORDER) c.sweep = 0;
   derive_grid(h->nb, c);
   h->kc[kind] = c;
   return 0;
+}
+
+// The row order in the chosen configurations after the pattern or the
+// FKPM_OPT_ROW_ORDER switch changed: off where the pattern has none, forced
+// on or off by the switch, else as chosen (its sweep is one range per XCD).
+void apply_row_order(fkpm_handle* h) {
+  for (int k = 0; k < NKIND; k++) {
+    KCfg& c = h->kc[k];
+    if (c.L <= 0) continue;
+    const bool can = h->has_order && !c.mfma && k != K_OUT;
+    if (!can || h->order_force == 0) c.nt &= ~fkd::NT_ORDER;
+    else if (h->order_force == 1) c.nt |= fkd::NT_ORDER;
+    if (c.nt & fkd::NT_ORDER) c.sweep = 0;
+    derive_grid(h->nb, c);
+  }
+}
+
+// The row array a step kernel of configuration c reads: the row offsets, or
+// the row order's descriptors (NT_ORDER).
+const int* rp_for(const fkpm_handle* h, const KCfg& c) {
+  return (c.nt & fkd::NT_ORDER) ? h->rdesc.as<const int>() : h->rp.as<const int>();
 }
 
 // Kernel entry of a kind under its configuration.
@@ -821,16 +850,22 @@ int tune_t(fkpm_handle* h) {
       const int nsl = h->qp / w >= 2 ? 2 : 1;
       for (int cap : caps)
         for (int nt : nts)
-          for (int sl = 0; sl < nsl; sl++)
+          for (int sl = 0; sl < nsl; sl++) {
+            KCfg c;
+            c.L = gs[g].L;
+            c.KQ = gs[g].KQ;
+            c.maxwg = cap;
+            c.nt = nt | (sl ? fkd::NT_SLICE : 0);
             for (int sw : sweeps) {
-              KCfg c;
-              c.L = gs[g].L;
-              c.KQ = gs[g].KQ;
-              c.maxwg = cap;
-              c.nt = nt | (sl ? fkd::NT_SLICE : 0);
               c.sweep = sw;
               cands.push_back(c);
             }
+            if (h->has_order && kind != K_OUT) {  // the lattice row order, one range per XCD
+              c.nt |= fkd::NT_ORDER;
+              c.sweep = 0;
+              cands.push_back(c);
+            }
+          }
     }
     if (mfma_avail(h, kind))
       for (int v = 1; v <= (mfma_rr_avail(h, kind) ? 2 : 1); v++)
@@ -852,7 +887,7 @@ int tune_t(fkpm_handle* h) {
           auto run = [&]() -> int {
             if (kind == K_MOMT) {  // alpha_{t+2} = 2 Hs alpha_{t+1} - alpha_t into a fresh slot
               const int t = tstep++;
-              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
                             (const S*)slot((t + 1) % ns), (const S*)slot(t % ns),
                             slot((t + 2) % ns), h->nb, h->qp, k.chunk, 2.0, -1.0,
                             h->partial.as<double>(), k.nt, no_rsource<S>());
@@ -860,7 +895,7 @@ int tune_t(fkpm_handle* h) {
             if (kind == K_ADS) {  // backward over slots ns-2 .. 1, beta in the ring
               const int t = tstep++;
               const int m = ns - 2 - t % (ns - 2);
-              return launch(ks.ad_stored, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+              return launch(ks.ad_stored, dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
                             (const S*)slot(m), (const S*)slot(m + 1),
                             (const S*)bring[(t + 1) % 3], (const S*)bring[t % 3],
                             (const S*)slot(m - 1), bring[(t + 2) % 3], h->tune_dacc.as<double>(),
@@ -872,16 +907,16 @@ int tune_t(fkpm_handle* h) {
             S* y = ring[(t + 2) % 3];
             // recurrences keep |x| bounded: y = 2 Hs x - z stays within the spectrum scale
             if (kind == K_MOM)
-              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
                             (const S*)x, (const S*)z, y, h->nb, h->qp, k.chunk, 2.0, -1.0,
                             h->partial.as<double>(), k.nt, no_rsource<S>());
             if (kind == K_XI)
-              return launch(ks.xi, dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, (const S*)x,
+              return launch(ks.xi, dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv, (const S*)x,
                             (const S*)z, y, h->xi.as<double>(), h->nb, h->qp, k.chunk, 2.0, -1.0,
                             0.0, 0.5, t == 0 ? 1 : 0, k.nt);
             // backward step: reads alpha_m, alpha_{m+1}, beta_{m+1}, beta_{m+2};
             // writes alpha_{m-1}, beta_m into the third buffers
-            return launch(adf_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp, ci, hv,
+            return launch(adf_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
                           (const S*)x, (const S*)z, (const S*)bring[(t + 1) % 3],
                           (const S*)bring[t % 3], y, bring[(t + 2) % 3], h->tune_dacc.as<double>(),
                           h->nb, h->qp, k.chunk, 0.1, 0.1, 0.1, 1.0, 0.0, 2.0, k.nt);
@@ -962,7 +997,7 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
       const S* x = (const S*)h->vec_a(m);
       const S* z = m == 0 ? nullptr : (const S*)h->vec_a(m - 1);
       S* y = (S*)h->vec_a(m + 1);
-      CHK(launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(),
+      CHK(launch(mom_fn(ks, kc), grid, block, h->stream, rp_for(h, kc),
                  h->ci.as<const int>(), h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk,
                  m == 0 ? 1.0 : 2.0, m == 0 ? 0.0 : -1.0,
                  h->partial.as<double>() + (size_t)m * kc.nwg * 2, kc.nt, no_rsource<S>()));
@@ -1039,7 +1074,7 @@ int stoch_clenshaw(fkpm_handle* h, int M, const double* c, const S** xi_out) {
   Vb b1{R, c[M - 1]}, b2{nullptr, 0.0};
   int slot = 0, launches = 0;
   auto step = [&](double xfac, double cr, S* y) -> int {
-    return launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(), h->ci.as<const int>(),
+    return launch(mom_fn(ks, kc), grid, block, h->stream, rp_for(h, kc), h->ci.as<const int>(),
                   h->hv.as<const S>(), b1.p, b2.p, y, h->nb, h->qp, kc.chunk, xfac * b1.s,
                   b2.p ? -b2.s : 0.0, (double*)nullptr, kc.nt, rsource<S>(h, cr));
   };
@@ -1104,7 +1139,7 @@ int stoch_table(fkpm_handle* h, int M, const double* c, const S** xi_out) {
     const bool last = (k == K - 1);
     const double cx = (k == 0 ? 1.0 : 2.0) * (last ? 2.0 : 1.0);
     const double cz = k == 0 ? 0.0 : (last ? -2.0 : -1.0);
-    CHK(launch(mom_fn(ks, kc), grid, block, h->stream, h->rp.as<const int>(),
+    CHK(launch(mom_fn(ks, kc), grid, block, h->stream, rp_for(h, kc),
                h->ci.as<const int>(), h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk, cx,
                cz, (double*)nullptr, kc.nt, last ? addv : no_rsource<S>()));
     z = x;
@@ -1156,12 +1191,13 @@ int stoch_t(fkpm_handle* h, int M, const double* c) {
     {
       LoopRecorder rec(h);
       // alpha_1 = Hs R, xi = c0 R + c1 alpha_1
-      CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(0),
+      CHK(launch(ksx.xi, grid, block, h->stream, rp_for(h, kx), ci, hv, (const S*)h->vec_a(0),
                  (const S*)nullptr, (S*)h->vec_a(1), h->xi.as<double>(), h->nb, h->qp, kx.chunk,
                  1.0, 0.0, c[0], c[1], 1, kx.nt));
       for (int m = 2; m < M; m++) {
         // alpha_m = 2 Hs alpha_{m-1} - alpha_{m-2}; xi += c_m alpha_m
-        CHK(launch(ksx.xi, grid, block, h->stream, rp, ci, hv, (const S*)h->vec_a(m - 1),
+        CHK(launch(ksx.xi, grid, block, h->stream, rp_for(h, kx), ci, hv,
+                   (const S*)h->vec_a(m - 1),
                    (const S*)h->vec_a(m - 2), (S*)h->vec_a(m), h->xi.as<double>(), h->nb, h->qp,
                    kx.chunk, 2.0, -1.0, 0.0, c[m], 0, kx.nt));
       }
@@ -1235,16 +1271,18 @@ int autodiff_t(fkpm_handle* h, int M, const double* c) {
           CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, am, first ? am : bm1,
                      dacc, h->nb, h->qp, ko.chunk, first ? 2.0 * sb : 2.0));
         CHK(launch(h->d_fused ? kst.ad_stored : kst.ad_stored_plain, dim3(kt.nwg), block,
-                   h->stream, rp, ci, hv, am, am1, bm1, bm2, (const S*)aout, bout, dacc, h->nb,
-                   h->qp, kt.chunk, c4, c2, w, sb, sb2, 2.0, kt.nt));
+                   h->stream, rp_for(h, kt), ci, hv, am, am1, bm1, bm2, (const S*)aout, bout, dacc,
+                   h->nb, h->qp, kt.chunk, c4, c2, w, sb, sb2, 2.0, kt.nt));
       } else if (h->d_fused) {
-        CHK(launch(adf_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
+        CHK(launch(adf_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp_for(h, ka), ci, hv, am, am1,
+                   bm1, bm2,
                    aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
       } else {
         // D += 2 alpha_m beta_{m+1}^+ on D's own pattern, then the step.
         CHK(launch(kso.outer_ss, dim3(ko.nwg), block, h->stream, drp, dci, am, first ? am : bm1,
                    dacc, h->nb, h->qp, ko.chunk, first ? 2.0 * sb : 2.0));
-        CHK(launch(adp_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp, ci, hv, am, am1, bm1, bm2,
+        CHK(launch(adp_fn(ksa, ka), dim3(ka.nwg), block, h->stream, rp_for(h, ka), ci, hv, am, am1,
+                   bm1, bm2,
                    aout, bout, dacc, h->nb, h->qp, ka.chunk, c4, c2, w, sb, sb2, 2.0, ka.nt));
       }
       launches++;
@@ -1470,8 +1508,9 @@ int moments2_t(fkpm_handle* h, int M, int col_chunk, size_t budget, double* mu) 
   const dim3 grid(c.nwg), block(fkd::WG);
   auto step = [&](const int* rp_, const int* ci_, const S* hv_, const S* x, const S* z, S* y,
                   double cx, double cz) {
+    // the row order belongs to H's pattern; the currents' products sweep rows
     return launch(ks.mom, grid, block, h->stream, rp_, ci_, hv_, x, z, y, h->nb, qpc, c.chunk,
-                  cx, cz, (double*)nullptr, c.nt, no_rsource<S>());
+                  cx, cz, (double*)nullptr, c.nt & ~fkd::NT_ORDER, no_rsource<S>());
   };
   const int* j1rp = h->j_rp[0].as<const int>();
   const int* j1ci = h->j_ci[0].as<const int>();
@@ -1531,6 +1570,57 @@ int upload_H_s(fkpm_handle* h, DevBuf& dst, const void* val, double mag, double 
 }
 
 // Pattern upload shared by set_H and energy_scale.
+// A periodic 3-D lattice in natural order (block row x + n1 (y + n2 z)),
+// recognised from H's block offsets: the three distances |j - i| (mod nb, the
+// shorter way round) that at least half the rows have are 1 < n1 < n1 n2 with
+// n1 n2 n3 = nb and every side at least 8.  Fills n[3]; false otherwise.
+bool lattice_dims(int nb, const int* rp, const int* ci, int n[3]) {
+  if (nb < 512) return false;
+  std::map<int, long long> cnt;
+  for (int i = 0; i < nb; i++)
+    for (int p = rp[i]; p < rp[i + 1]; p++) {
+      int d = ci[p] - i;
+      if (d == 0) continue;
+      if (d < 0) d += nb;
+      cnt[std::min(d, nb - d)]++;
+    }
+  std::vector<int> dist;
+  for (const auto& kv : cnt)
+    if (2 * kv.second >= nb) dist.push_back(kv.first);
+  if (dist.size() != 3 || dist[0] != 1) return false;
+  const long long n1 = dist[1], c = dist[2];
+  if (c % n1 != 0 || nb % c != 0) return false;
+  n[0] = (int)n1;
+  n[1] = (int)(c / n1);
+  n[2] = (int)(nb / c);
+  return n[0] >= 8 && n[1] >= 8 && n[2] >= 8;
+}
+
+// Sweep order for an n1 x n2 x n3 lattice: one x-slab of n1 / 8 sites per XCD
+// (positions [x nb/8, (x+1) nb/8) of the sweep), each walked in y-tiles of
+// ty rows (about 128 sites a plane of the tile) and, inside a tile, plane by
+// plane along z.  A row's +-z neighbours are then one tile-plane away in the
+// sweep instead of a whole slab-plane (on 64^3: 128 instead of 512 sites), so
+// the neighbour rows of both vectors an autodiff step gathers stay in the
+// XCD's L2.  Requires n1 % 8 == 0.
+bool lattice_order(int nb, const int* rp, const int* ci, std::vector<int>& order) {
+  int n[3];
+  if (!lattice_dims(nb, rp, ci, n) || n[0] % 8 != 0) return false;
+  const int w = n[0] / 8;
+  int ty = 1;
+  for (int t = 1; t <= n[1]; t++)
+    if (n[1] % t == 0 && w * t <= 128) ty = t;
+  order.clear();
+  order.reserve(nb);
+  for (int x = 0; x < 8; x++)
+    for (int y0 = 0; y0 < n[1]; y0 += ty)
+      for (int z = 0; z < n[2]; z++)
+        for (int yo = 0; yo < ty; yo++)
+          for (int xo = 0; xo < w; xo++)
+            order.push_back(x * w + xo + n[0] * ((y0 + yo) + n[1] * z));
+  return (int)order.size() == nb;
+}
+
 int upload_pattern(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr,
                    const int* col_idx) {
   h->nb = nb;
@@ -1553,7 +1643,22 @@ int upload_pattern(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr,
                         hipMemcpyHostToDevice, h->stream));
   const size_t hvb = (size_t)nnzb * b * b * host_elem_bytes(h->dtype);
   CHK(h->hraw.ensure(hvb > 0 ? hvb : 16));
-  // the async copies above read host arrays owned by the caller: finish them
+  std::vector<int> order, desc;
+  h->has_order = lattice_order(nb, row_ptr, col_idx, order);
+  if (h->has_order) {
+    desc.resize(3 * (size_t)nb);
+    for (int k = 0; k < nb; k++) {
+      desc[3 * (size_t)k] = order[k];
+      desc[3 * (size_t)k + 1] = row_ptr[order[k]];
+      desc[3 * (size_t)k + 2] = row_ptr[order[k] + 1];
+    }
+    CHK(h->rdesc.ensure(sizeof(int) * desc.size()));
+    HIPCHK(hipMemcpyAsync(h->rdesc.p, desc.data(), sizeof(int) * desc.size(),
+                          hipMemcpyHostToDevice, h->stream));
+  }
+  apply_row_order(h);
+  // the async copies above read host arrays owned by the caller (and desc):
+  // finish them
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -2127,7 +2232,7 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
       if (value >= 0)
         for (int k = 0; k < NKIND; k++)
           if (h->kc[k].L > 0) {
-            h->kc[k].sweep = value;
+            h->kc[k].sweep = (h->kc[k].nt & fkd::NT_ORDER) ? 0 : value;
             derive_grid(h->nb, h->kc[k]);
           }
       return 0;
@@ -2142,6 +2247,11 @@ int fkpm_set_option(fkpm_handle* h, int option, int value) {
         for (int k = 0; k < NKIND; k++)
           if (h->kc[k].L > 0 && !h->kc[k].mfma)
             h->kc[k].nt = (h->kc[k].nt & ~fkd::NT_SLICE) | (value ? fkd::NT_SLICE : 0);
+      return 0;
+    case FKPM_OPT_ROW_ORDER:
+      if (value < -1 || value > 1) return fail(FKPM_E_ARG, "bad row-order switch %d", value);
+      h->order_force = value;
+      apply_row_order(h);
       return 0;
     case FKPM_OPT_ALPHA_TABLE_MB:
       if (value < -1) return fail(FKPM_E_ARG, "bad table budget %d MiB", value);

@@ -119,6 +119,13 @@ constexpr int NT_STORE = 2;  // stores of the step's outputs
 // next plane) take 1/passes of the bytes and stay in its L2, at the cost of
 // reading H's indices and values once per pass.
 constexpr int NT_SLICE = 4;
+// Row order: `rp` holds three ints per position k of the sweep, {row, first
+// block, end block}, instead of the row offsets, and the workgroups sweep the
+// positions (one contiguous range per XCD) rather than the rows.  The host
+// builds it for a periodic 3-D lattice so that each XCD walks its slab tile by
+// tile and a row's far neighbours (the next plane) are still in its L2.  A
+// row's arithmetic is the same in either order.
+constexpr int NT_ORDER = 8;
 
 // DPP lane move inside 16-lane rows (gfx9 encodings: quad_perm 0x00-0xFF,
 // row_mirror 0x140, row_half_mirror 0x141).
@@ -346,14 +353,18 @@ __device__ __forceinline__ RowRange wg_rows(int nb, int chunk, int step, WgId wi
   return rr;
 }
 
-// Block row offsets rp[i], rp[i+1] (zeros past r1).  The row loops load the
-// next row's pair before working on the current row.
+// The block row at sweep position i and its block range: row i and rp[i],
+// rp[i+1], or with NT_ORDER the descriptor rp[3i .. 3i+2] (empty past r1).
+// The row loops load the next position's entry before working on the current
+// row.
 struct RowPtr {
-  int p0, p1;
+  int row, p0, p1;
 };
 
-__device__ __forceinline__ RowPtr row_ptr(const int* __restrict__ rp, int i, int r1) {
-  return i < r1 ? RowPtr{rp[i], rp[i + 1]} : RowPtr{0, 0};
+__device__ __forceinline__ RowPtr row_ptr(const int* __restrict__ rp, int i, int r1, int nt) {
+  if (i >= r1) return RowPtr{i, 0, 0};
+  if (nt & NT_ORDER) return RowPtr{rp[3 * i], rp[3 * i + 1], rp[3 * i + 2]};
+  return RowPtr{i, rp[i], rp[i + 1]};
 }
 
 // A group's first row; with one group per wave (L = 64) the index is made
@@ -583,13 +594,15 @@ __device__ __forceinline__ void step_moments_dev(
 
   const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
   for (int qs = 0; qs < qp; qs += qw) {
-  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1, nt);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const RowPtr cur = nxt;
-    nxt = row_ptr(rp, i + rr.stride, rr.r1);
+    nxt = row_ptr(rp, i + rr.stride, rr.r1, nt);
     if (i >= rr.r1) continue;
+    i = cur.row;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
@@ -688,13 +701,15 @@ __global__ __launch_bounds__(WG) void k_step_xi(
 
   const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
   for (int qs = 0; qs < qp; qs += qw) {
-  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1, nt);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const RowPtr cur = nxt;
-    nxt = row_ptr(rp, i + rr.stride, rr.r1);
+    nxt = row_ptr(rp, i + rr.stride, rr.r1, nt);
     if (i >= rr.r1) continue;
+    i = cur.row;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
@@ -842,13 +857,15 @@ __device__ __forceinline__ void step_autodiff_dev(
 
   const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
   for (int qs = 0; qs < qp; qs += qw) {
-  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1, nt);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const RowPtr cur = nxt;
-    nxt = row_ptr(rp, i + rr.stride, rr.r1);
+    nxt = row_ptr(rp, i + rr.stride, rr.r1, nt);
     if (i >= rr.r1) continue;
+    i = cur.row;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;
@@ -1019,13 +1036,15 @@ __device__ __forceinline__ void step_autodiff_stored_dev(
 
   const int qw = (nt & NT_SLICE) ? L * KQ : qp;  // columns per sweep over the rows
   for (int qs = 0; qs < qp; qs += qw) {
-  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1);
+  RowPtr nxt = row_ptr(rp, first_row<L>(rr.r0 + g), rr.r1, nt);
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     int i = base + g;
     if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const RowPtr cur = nxt;
-    nxt = row_ptr(rp, i + rr.stride, rr.r1);
+    nxt = row_ptr(rp, i + rr.stride, rr.r1, nt);
     if (i >= rr.r1) continue;
+    i = cur.row;
+    if constexpr (L == WAVE) i = __builtin_amdgcn_readfirstlane(i);
     const int p0 = cur.p0, p1 = cur.p1;
     for (int q0 = qs; q0 < qs + qw; q0 += L * KQ) {
       const size_t qo = (size_t)(q0 + l) * NS;

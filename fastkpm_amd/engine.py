@@ -316,6 +316,12 @@ class HipEngine:
         back to back, -1 the configuration's own."""
         call("fkpm_set_option", self._h, _capi.OPT_SLICE, int(mode))
 
+    def set_row_order(self, mode: int) -> None:
+        """Lattice row order of the vector-unit step kernels
+        (FKPM_OPT_ROW_ORDER): 1 slab by slab in y-tiles along z when H is a
+        periodic 3-D lattice, 0 row order, -1 the configuration's own."""
+        call("fkpm_set_option", self._h, _capi.OPT_ROW_ORDER, int(mode))
+
     def set_graph(self, mode: int) -> None:
         """Step loops as hipGraphs (FKPM_OPT_GRAPH): 1 always, 0 never, -1
         automatic (short steps)."""

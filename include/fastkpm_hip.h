@@ -209,6 +209,16 @@ int fkpm_moments_noalloc(int device, int prec, int nb, int b, int nnzb, const in
                                    2^22 scalars (steps short enough for launches to set the
                                    time); the environment variable FKPM_GRAPH (-1, 0 or 1) at
                                    fkpm_dev_open sets the handle's default */
+#define FKPM_OPT_ROW_ORDER 9    /* value 1: when H's pattern is a periodic 3-D lattice in
+                                   natural order (block row x + n1 (y + n2 z), every side at
+                                   least 8, n1 a multiple of 8; recognised at fkpm_set_H from
+                                   the block offsets), the vector-unit step kernels visit the
+                                   rows slab by slab (one x-slab per XCD) and, in a slab, in
+                                   y-tiles along z, so the next plane's rows are still in the
+                                   XCD's L2; results per row are the same as in row order, the
+                                   moment sums add in another order; 0: row order; -1
+                                   (default): the configuration's own (the tuner's choice,
+                                   else 0).  Patterns of another shape keep row order. */
 #define FKPM_OPT_ALPHA_TABLE_MB 7 /* value >= 0: the most MiB the table of forward vectors
                                    (FKPM_OPT_STORE_ALPHA) may hold on this handle; -1
                                    (default): a quarter of the device's memory and at most

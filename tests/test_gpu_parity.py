@@ -452,6 +452,66 @@ def test_column_slices_outer_loop(oracle, prec):
             eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lattice_row_order(oracle, prec):
+    """FKPM_OPT_ROW_ORDER 1 on a periodic 3-D lattice (16 x 8 x 8 sites, 2x2
+    blocks): the step kernels visit the rows slab by slab (one x-slab of two
+    sites per XCD) in y-tiles along z.  Moments, dE/dH and stoch_matrix (the
+    Clenshaw form, and the forward form with the kept table) against the
+    oracle, and against row order: D bitwise the same (a row's arithmetic does
+    not depend on when it is visited), the moments up to the order of the
+    dot-product sums."""
+    H, R, es, orc = _pair(oracle, shape=(16, 8, 8), J=1.0, W=1.0)
+    M = 40
+    c = fermi_coeffs(oracle, M, es.lo, es.hi)
+    mu_o, d_o, s_o = orc.moments(M), orc.autodiff_matrix(c, H), orc.stoch_matrix(c, H)
+    tol, tol_d = (FP64_TOL, FP64_TOL) if prec == "fp64" else (2e-5, 1e-4)
+    for L, KQ in [(64, 1), (16, 2), (4, 1)]:
+        out = {}
+        for order in (0, 1):
+            for store, method in ((0, "clenshaw"), (2, "forward")):
+                eng = _engine("cx_double", prec)
+                eng.set_R(R)
+                eng.set_H(H, es)
+                eng.set_geometry(L, KQ, 0)
+                eng.set_row_order(order)
+                eng.set_store_alpha(store)
+                eng.set_stoch_method(method)
+                mu = eng.moments(M)
+                D = H.like()
+                eng.autodiff_matrix(c, D)
+                S = H.like()
+                eng.stoch_matrix(c, S)
+                for k in (0, 1, 2, 4, 5):
+                    kc = eng.kernel_config(k)
+                    assert bool(kc["nt"] & 8) is bool(order), (k, kc)
+                    if order:
+                        assert kc["sweep"] == 0
+                assert not eng.kernel_config(3)["nt"] & 8
+                eng.close()
+                assert rel_err(mu, mu_o) < tol, (L, KQ, order, store)
+                assert rel_err(D.val, d_o) < tol_d, (L, KQ, order, store)
+                assert rel_err(S.val, s_o) < tol_d, (L, KQ, order, method)
+                out[order, store] = (mu, D.val.copy(), S.val.copy())
+        for store in (0, 2):
+            assert np.array_equal(out[0, store][1], out[1, store][1]), (L, KQ, store)
+            assert np.array_equal(out[0, store][2], out[1, store][2]), (L, KQ, store)
+            assert rel_err(out[0, store][0], out[1, store][0]) < (1e-13 if prec == "fp64" else 1e-6)
+
+
+def test_row_order_needs_a_3d_lattice(oracle):
+    """A 2-D lattice has no row order: the switch leaves row order in place."""
+    H, R, es, orc = _pair(oracle, shape=(16, 16), J=1.0)
+    eng = _engine()
+    eng.set_R(R)
+    eng.set_H(H, es)
+    eng.set_row_order(1)
+    mu = eng.moments(20)
+    assert not eng.kernel_config(0)["nt"] & 8
+    assert rel_err(mu, orc.moments(20)) < FP64_TOL
+    eng.close()
+
+
 @pytest.mark.parametrize("prec,store", [("fp64", 1), ("fp32", 1), ("fp64", 2), ("fp32", 2)])
 def test_autotuned_engine(oracle, prec, store):
     """store 2: the forward vectors are kept, and the tuner also times the
