@@ -248,6 +248,8 @@ struct KCfg {
                  // 1 general, 2 row-resident (rows of <= 8 blocks, vectors < 4 GiB)
   int sweep = 0;  // rows: 0 per-XCD regions, s >= 1 one front in segments of s passes
                   // per XCD (chunk = -s, workgroups a multiple of 8 s)
+  int dm = 8;     // row-resident matrix-core kernels: block slots per row (7 when no
+                  // row has more than 7 blocks, else 8); set by set_kind
   int nwg = 0, chunk = 0;
 };
 
@@ -492,6 +494,7 @@ int set_kind(fkpm_handle* h, int kind, KCfg c) {
     c.L = 64;  // one wave per block row
     c.KQ = 1;
     c.nt = 0;
+    c.dm = h->max_deg <= 7 ? 7 : 8;
   }
   if (!lookup_kset(h, kind, h->b, c.L, c.KQ))
     return fail(FKPM_E_UNSUP, "no kernel for block %d, lanes %d, packets %d", h->b, c.L, c.KQ);
@@ -532,11 +535,12 @@ const int* rp_for(const fkpm_handle* h, const KCfg& c) {
 // Kernel entry of a kind under its configuration.
 template <class S>
 auto mom_fn(const fkd::KSet<S>& ks, const KCfg& c) {
-  return c.mfma == 2 ? ks.mfma_mom_rr : c.mfma ? ks.mfma_mom : ks.mom;
+  return c.mfma == 2 ? (c.dm == 7 ? ks.mfma_mom_rr7 : ks.mfma_mom_rr) : c.mfma ? ks.mfma_mom : ks.mom;
 }
 template <class S>
 auto adf_fn(const fkd::KSet<S>& ks, const KCfg& c) {
-  return c.mfma == 2 ? ks.mfma_ad_rr : c.mfma ? ks.mfma_ad_fused : ks.ad_fused;
+  return c.mfma == 2 ? (c.dm == 7 ? ks.mfma_ad_rr7 : ks.mfma_ad_rr)
+                      : c.mfma ? ks.mfma_ad_fused : ks.ad_fused;
 }
 template <class S>
 auto adp_fn(const fkd::KSet<S>& ks, const KCfg& c) {
@@ -600,7 +604,8 @@ std::map<std::string, std::array<KCfg, NKIND>> g_tune_cache;
 std::string tune_key(const fkpm_handle* h) {
   char k[128];
   snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb,
-           h->nnzb, h->mfma_on ? 1 : 0, h->max_deg <= kRowResidentMax ? 1 : 0, table_wanted(h) ? 1 : 0);
+           h->nnzb, h->mfma_on ? 1 : 0,
+           h->max_deg <= 7 ? 2 : h->max_deg <= kRowResidentMax ? 1 : 0, table_wanted(h) ? 1 : 0);
   return k;
 }
 
@@ -716,7 +721,7 @@ int ensure_ready(fkpm_handle* h) {
   // backstop: a row-resident matrix-core kernel on a pattern with longer rows
   // would read only the first kRowResidentMax blocks of each row
   for (int k = 0; k < NKIND && !stale; k++)
-    stale = h->kc[k].mfma == 2 && !mfma_rr_avail(h, k);
+    stale = h->kc[k].mfma == 2 && (!mfma_rr_avail(h, k) || (h->kc[k].dm == 7 && h->max_deg > 7));
   if (!stale) return 0;
   CHK(choose_geometry(h));
   const size_t vb = h->vbytes();
@@ -1926,12 +1931,14 @@ int fkpm_set_H(fkpm_handle* h, int nb, int b, int nnzb, const int* row_ptr, cons
   }
   // The kernel choice depends on the block size, the block-row and block
   // counts (grids, tuned-shape key) and on whether every row has at most
-  // kRowResidentMax blocks (row-resident matrix-core kernels read only that many): a
-  // change of any of them picks the kernels again at the next use.
+  // kRowResidentMax blocks (row-resident matrix-core kernels read only that many),
+  // or at most 7 (their instantiation without a padding slot): a change of any
+  // of them picks the kernels again at the next use.
   const int old_b = h->b, old_nb = h->nb, old_nnzb = h->nnzb, old_deg = h->max_deg;
   CHK(upload_pattern(h, nb, b, nnzb, row_ptr, col_idx));
   const bool geo_change = b != old_b || nb != old_nb || nnzb != old_nnzb ||
-                          (old_deg <= kRowResidentMax) != (h->max_deg <= kRowResidentMax);
+                          (old_deg <= kRowResidentMax) != (h->max_deg <= kRowResidentMax) ||
+                          (old_deg <= 7) != (h->max_deg <= 7);
   h->avg = avg;
   h->mag = mag;
   const size_t eb = h->single ? 4 : 8;

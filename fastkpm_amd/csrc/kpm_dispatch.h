@@ -55,6 +55,12 @@ struct KSet {
   void (*mfma_ad_rr)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
                      S*, S*, double*, int, int, int, double, double, double, double, double,
                      double, int) = nullptr;
+  // the same for rows of at most 7 blocks (no padding slot)
+  void (*mfma_mom_rr7)(const int*, const int*, const S*, const S*, const S*, S*, int, int, int,
+                       double, double, double*, int, RSource<S>) = nullptr;
+  void (*mfma_ad_rr7)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
+                      S*, S*, double*, int, int, int, double, double, double, double, double,
+                      double, int) = nullptr;
   int B = 0, L = 0, KQ = 0;
 };
 

@@ -386,11 +386,13 @@ __device__ __forceinline__ void mfma_step_rr_dev(
         C2 x0[G], x1[G];
 #pragma unroll
         for (int v = 0; v < G; v++) {
+          if (u0 + v >= DM) continue;  // DM % G != 0: the last group is short
           x0[v] = IO::ld(bx, o0[u0 + v], so);
           x1[v] = IO::ld(bx, o0[u0 + v] + half, so);
         }
 #pragma unroll
-        for (int v = 0; v < G; v++) acc = mfma_block<S>(a[u0 + v], x0[v], x1[v], acc);
+        for (int v = 0; v < G; v++)
+          if (u0 + v < DM) acc = mfma_block<S>(a[u0 + v], x0[v], x1[v], acc);
       }
       const C2 xo0 = IO::ld(bx, f0, so), xo1 = IO::ld(bx, f0 + half, so);
       const C2 z0 = IO::ld(bz, f0, so), z1 = IO::ld(bz, f0 + half, so);  // 0 without z
@@ -550,6 +552,7 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
         S qv[GR][4];
 #pragma unroll
         for (int v = 0; v < GR; v++) {
+          if (u0 + v >= DM) continue;  // DM % GR != 0: the last group is short
           xa0[v] = IO::ld(ba, o0[u0 + v], so);
           xa1[v] = IO::ld(ba, o0[u0 + v] + half, so);
           xb0[v] = IO::ld(bb, o0[u0 + v], so);
@@ -561,6 +564,7 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
         }
 #pragma unroll
         for (int v = 0; v < GR; v++) {
+          if (u0 + v >= DM) continue;
           accA = mfma_block<S>(a[u0 + v], xa0[v], xa1[v], accA);
           accB = mfma_block<S>(a[u0 + v], xb0[v], xb1[v], accB);
           if constexpr (FUSED_D) {
@@ -822,24 +826,26 @@ __global__ __launch_bounds__(WG) void k_mfma_autodiff(
   mfma_autodiff_dev<S, FUSED_D>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out, dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
 }
 
-template <typename S>
+// DM = 8 (MF_DM), or 7 for rows of at most 7 blocks (a cubic lattice's six
+// neighbours and the site: no padding slot in the matrix-core work).
+template <typename S, int DM>
 __global__ __launch_bounds__(WG) void k_mfma_step_rr(
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ x, const S* __restrict__ z, S* __restrict__ y, int nb, int qp,
     int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<S> rsrc) {
-  mfma_step_rr_dev<S, MF_DM, 4>(wg_self(), rp, ci, hv, x, z, y, nb, qp, chunk, cx, cz, partial, nt,
-                                rsrc);
+  mfma_step_rr_dev<S, DM, 4>(wg_self(), rp, ci, hv, x, z, y, nb, qp, chunk, cx, cz, partial, nt,
+                             rsrc);
 }
 
-template <typename S>
+template <typename S, int DM>
 __global__ __launch_bounds__(WG) void k_mfma_autodiff_rr(
     const int* __restrict__ rp, const int* __restrict__ ci, const S* __restrict__ hv,
     const S* __restrict__ am, const S* __restrict__ am1, const S* __restrict__ bm1,
     const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
     double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
     double sb, double sb2, double dscale, int nt) {
-  mfma_autodiff_rr_dev<S, true, MF_DM, 1>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out,
-                                          dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
+  mfma_autodiff_rr_dev<S, true, DM, 1>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out,
+                                       dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
 }
 
 }  // namespace fkd
