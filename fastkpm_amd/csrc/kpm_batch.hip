@@ -81,7 +81,8 @@ __device__ __forceinline__ S* vec_a(const SlotDev& d, int k) {
 
 // Forward Chebyshev step of every slot of the class in its forward phase.
 // MF: 0 vector units, 1 matrix cores, 2 matrix cores row-resident (rows of
-// at most MF_DM blocks, vectors below 4 GiB).
+// at most MF_DM blocks, vectors below 4 GiB), 3 the same for rows of at most
+// 7 blocks (no padding slot).
 template <typename S, int B, int L, int KQ, int MF>
 __global__ __launch_bounds__(WG) void k_batch_fwd(const SlotDev* __restrict__ sd,
                                                   const int* __restrict__ st,
@@ -100,9 +101,9 @@ __global__ __launch_bounds__(WG) void k_batch_fwd(const SlotDev* __restrict__ sd
   const double cx = step == 0 ? 1.0 : 2.0, cz = step == 0 ? 0.0 : -1.0;
   double* part = d.partial + (size_t)step * d.nwg * 2;
   RSource<S> nors{nullptr, nullptr, nullptr, 0.0};
-  if constexpr (MF == 2)
-    mfma_step_rr_dev<S, MF_DM, 4>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp, d.chunk,
-                                  cx, cz, part, 0, nors);
+  if constexpr (MF >= 2)
+    mfma_step_rr_dev<S, MF == 3 ? 7 : MF_DM, 4>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb,
+                                                d.qp, d.chunk, cx, cz, part, 0, nors);
   else if constexpr (MF == 1)
     mfma_step_dev<S>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp, d.chunk, cx, cz, part,
                      0, nors);
@@ -137,10 +138,10 @@ __global__ __launch_bounds__(WG) void k_batch_bwd(const SlotDev* __restrict__ sd
   S* bout = (S*)d.Bv[m % 3];
   const double c4 = 4 * d.c[2 * m], c2 = 2 * d.c[2 * m + 1];
   const double w = m > 1 ? 2 * d.c[2 * m - 1] : 0.0;
-  if constexpr (MF == 2 && sizeof(S) == 8)
-    mfma_autodiff_rr_dev<S, true, MF_DM, 1>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2,
-                                            aout, bout, d.dacc, d.nb, d.qp, d.chunk, c4, c2, w,
-                                            d.sb, sb2, 2.0, 0);
+  if constexpr (MF >= 2 && sizeof(S) == 8)
+    mfma_autodiff_rr_dev<S, true, MF == 3 ? 7 : MF_DM, 1>(
+        wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2, aout, bout, d.dacc, d.nb, d.qp,
+        d.chunk, c4, c2, w, d.sb, sb2, 2.0, 0);
   else if constexpr (MF >= 1)
     mfma_autodiff_dev<S, true>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2, aout, bout,
                                d.dacc, d.nb, d.qp, d.chunk, c4, c2, w, d.sb, sb2, 2.0, 0);
@@ -335,17 +336,19 @@ ClassDef mk_class(int W) {
 // FP64 or two FP32 complex columns; the narrowest class keeps a 16-column
 // FP32 request from being padded to 32); complex 8x8 blocks: matrix cores
 // (tiles of 16 columns) with the outer products at width 16 (FP64) or 8
-// (FP32) packets, general and row-resident (the last entry, taken for
-// requests whose rows all have at most MF_DM blocks).
+// (FP32) packets: general, row-resident on 8 block slots (requests whose rows
+// all have at most MF_DM blocks) and on 7 (at most 7 blocks: 3-D lattices).
 const std::vector<ClassDef>& classes(bool single) {
   static const std::vector<ClassDef> d = {
       mk_class<double, 2, 4, 2, 0>(8),    mk_class<double, 2, 8, 2, 0>(16),
       mk_class<double, 2, 16, 2, 0>(32),  mk_class<double, 2, 64, 1, 0>(64),
-      mk_class<double, 8, 16, 1, 1>(16),  mk_class<double, 8, 16, 1, 2>(16)};
+      mk_class<double, 8, 16, 1, 1>(16),  mk_class<double, 8, 16, 1, 2>(16),
+      mk_class<double, 8, 16, 1, 3>(16)};
   static const std::vector<ClassDef> f = {
       mk_class<float, 2, 4, 2, 0>(8),    mk_class<float, 2, 8, 2, 0>(16),
       mk_class<float, 2, 16, 2, 0>(32),  mk_class<float, 2, 64, 1, 0>(64),
-      mk_class<float, 8, 8, 1, 1>(8),    mk_class<float, 8, 8, 1, 2>(8)};
+      mk_class<float, 8, 8, 1, 1>(8),    mk_class<float, 8, 8, 1, 2>(8),
+      mk_class<float, 8, 8, 1, 3>(8)};
   return single ? f : d;
 }
 
@@ -427,13 +430,16 @@ struct Layout {
 enum Buf { B_RP, B_CI, B_RB, B_TP, B_HV, B_HRAW, B_R, B_A0, B_A1, B_A2, B_B0, B_B1, B_B2, B_RRAW,
            B_DACC, B_DOUT, B_PART, B_PSUM, B_C, B_COUNT };
 
-int pick_class(bool single, int b, int s, bool row_resident, int* W) {
+// mf: the matrix-core form a b = 8 request takes (1 general, 2 row-resident on
+// 8 slots, 3 on 7); vector units (0) for b = 2.
+int pick_class(bool single, int b, int s, int mf, int* W) {
   const auto& cl = classes(single);
   const int ne = single ? 2 : 1;  // complex elements per 16-byte packet
   const int need = (s + ne - 1) / ne;
+  const int want = b == 8 ? mf : 0;
   int best = -1;
   for (int k = 0; k < (int)cl.size(); k++) {
-    if (cl[k].B != b || (cl[k].mf == 2) != (b == 8 && row_resident)) continue;
+    if (cl[k].B != b || cl[k].mf != want) continue;
     if (best < 0) {
       best = k;
       continue;
@@ -457,7 +463,7 @@ void plan(bool single, const Request& r, Layout* L) {
   const int wb8 = single ? 8 : 16;
   const size_t vb8 = (size_t)r.n * ((need + wb8 - 1) / wb8) * wb8 * ne * 2 * es;
   const bool rr = r.b == 8 && r.max_deg <= fkd::MF_DM && vb8 < ((size_t)1 << 32);
-  L->cls = pick_class(single, r.b, r.s, rr, &L->W);
+  L->cls = pick_class(single, r.b, r.s, !rr ? 1 : r.max_deg <= 7 ? 3 : 2, &L->W);
   if (L->cls < 0) {
     L->total = 0;
     return;

@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import fermi_coeffs, rel_err, seeded_R
+from helpers import fermi_coeffs, rel_err, seeded_R, with_partner_bonds
 
 pytestmark = pytest.mark.gpu
 
@@ -183,10 +183,11 @@ def _dense_rows_b8(nb=20, extra=6, seed=3):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_batch_b8_row_resident_and_general(oracle, prec):
-    """8x8-block requests take the row-resident matrix-core class when every
-    row has at most 8 blocks and the general one otherwise; both in flight
-    together, each against the oracle."""
-    from fastkpm_amd import EnergyScale
+    """8x8-block requests take a row-resident matrix-core class when every
+    row has at most 8 blocks (on 7 block slots when no row has more than 7,
+    else on 8) and the general one otherwise; all three in flight together,
+    each against the oracle."""
+    from fastkpm_amd import EnergyScale, lattice
     from fastkpm_amd.serving import BatchEngine
     Hd = _dense_rows_b8()
     assert int(np.max(np.diff(Hd.row_ptr))) > 8
@@ -197,10 +198,18 @@ def test_batch_b8_row_resident_and_general(oracle, prec):
     Md = 28
     cd = fermi_coeffs(oracle, Md, esd.lo, esd.hi, kT=0.05)
     Hl, esl, cl, Rl, _ = _req(oracle, "b8", 43, 36)
+    shape8 = (4, 4, 4)
+    H8 = with_partner_bonds(lattice.lattice(shape8, b=8, orbital_blocks=True, seed=45), shape8, 46)
+    assert int(np.min(np.diff(H8.row_ptr))) == 8 and int(np.max(np.diff(H8.row_ptr))) == 8
+    lam8 = np.linalg.eigvalsh(H8.to_dense())
+    es8 = EnergyScale(lam8[0] - 0.2, lam8[-1] + 0.2)
+    c8 = fermi_coeffs(oracle, 20, es8.lo, es8.hi, kT=0.05)
+    R8 = seeded_R(oracle, lattice.colouring(shape8, (2, 2, 2), 8), 47)
     eng = BatchEngine(prec, hbm_bytes=1 << 30, max_slots=4)
-    ids = [eng.submit(Hd, esd, cd, R=Rd), eng.submit(Hl, esl, cl, R=Rl)]
+    reqs = [(Hd, esd, cd, Rd), (Hl, esl, cl, Rl), (H8, es8, c8, R8)]
+    ids = [eng.submit(H, es, c, R=R) for H, es, c, R in reqs]
     tol_mu, tol_d = (FP64_TOL, FP64_TOL) if prec == "fp64" else (2e-5, 1e-4)
-    for (H, es, c, R), rid in zip([(Hd, esd, cd, Rd), (Hl, esl, cl, Rl)], ids):
+    for (H, es, c, R), rid in zip(reqs, ids):
         r = eng.wait(rid)
         mu_o, D_o = _oracle_solve(oracle, H, es, c, R)
         assert rel_err(r.mu, mu_o) < tol_mu

@@ -24,7 +24,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import rel_err, seeded_R
+from helpers import rel_err, seeded_R, with_partner_bonds
 
 pytestmark = pytest.mark.gpu
 
@@ -218,24 +218,6 @@ def test_set_H_with_longer_rows_picks_kernels_again(oracle):
         assert rel_err(D2.val, orc.stoch_matrix(cF, D2)) < FP64_TOL, deg(H)
 
 
-def _with_partner_bonds(H1, shape, seed):
-    """H1 plus random Hermitian 8 x 8 blocks on the (+2, 0, 0) bond of a lattice
-    4 sites wide in x (+2 and -2 are the same site): 7 + 1 = 8 blocks per row."""
-    from fastkpm_amd import lattice
-    rng = np.random.default_rng(seed)
-    b = H1.b_len
-    nsite = H1.n_rows
-    idx = np.arange(nsite).reshape(shape, order="F")
-    src = idx.reshape(-1, order="F")
-    dst = np.roll(idx, -2, axis=0).reshape(-1, order="F")
-    T = (rng.standard_normal((nsite, b, b)) + 1j * rng.standard_normal((nsite, b, b))) * 0.3
-    rows = [H1.row_idx, src, dst]
-    cols = [H1.col_idx, dst, src]
-    blks = [H1.val.reshape(-1, b, b).transpose(0, 2, 1), T, np.conj(T.transpose(0, 2, 1))]
-    return lattice.coo_to_bsr(nsite, b, np.concatenate(rows), np.concatenate(cols),
-                              np.concatenate(blks, axis=0))
-
-
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_row_resident_seven_and_eight_block_rows(oracle, prec):
     """The row-resident matrix-core kernels come in two instantiations: 7 block
@@ -244,7 +226,7 @@ def test_row_resident_seven_and_eight_block_rows(oracle, prec):
     from fastkpm_amd import EnergyScale, HipEngine, lattice
     shape = (4, 4, 4)
     H7 = lattice.lattice(shape, b=8, orbital_blocks=True, seed=5)
-    H8 = _with_partner_bonds(H7, shape, 6)
+    H8 = with_partner_bonds(H7, shape, 6)
     deg = lambda H: int(np.max(np.diff(H.row_ptr)))
     assert deg(H7) == 7 and deg(H8) == 8 and int(np.min(np.diff(H8.row_ptr))) == 8
     groups = lattice.colouring(shape, (2, 2, 2), 8)
