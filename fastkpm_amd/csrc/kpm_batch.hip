@@ -139,7 +139,7 @@ __global__ __launch_bounds__(WG) void k_batch_bwd(const SlotDev* __restrict__ sd
   const double c4 = 4 * d.c[2 * m], c2 = 2 * d.c[2 * m + 1];
   const double w = m > 1 ? 2 * d.c[2 * m - 1] : 0.0;
   if constexpr (MF >= 2 && sizeof(S) == 8)
-    mfma_autodiff_rr_dev<S, true, MF == 3 ? 7 : MF_DM, 1>(
+    mfma_autodiff_rr_dev<S, true, MF == 3 ? 7 : MF_DM, mf_rr_group<S, MF == 3 ? 7 : MF_DM>()>(
         wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2, aout, bout, d.dacc, d.nb, d.qp,
         d.chunk, c4, c2, w, d.sb, sb2, 2.0, 0);
   else if constexpr (MF >= 1)

@@ -345,6 +345,9 @@ __device__ __forceinline__ void mfma_step_rr_dev(
   const __amdgpu_buffer_rsrc_t bz = vec_rsrc(z != nullptr ? z : x, z != nullptr ? vbytes : 0u);
   const S acx = (S)cx, acz = (S)cz;
   const uint32_t half = 4u * rs * ES;  // orbital q -> q + 4
+  // a neighbour's offset: this lane's constant part (VGPR) plus the block's
+  // wave-uniform 8 j rs ES (the buffer load's SGPR offset)
+  const uint32_t vlo = (m.q * rs + 2u * m.c) * ES;
   double dmm = 0.0, dm1 = 0.0;
 
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
@@ -352,7 +355,7 @@ __device__ __forceinline__ void mfma_step_rr_dev(
     if (i >= rr.r1) continue;
     const int p0 = rp[i], deg = rp[i + 1] - p0;
     S a[DM][4];
-    uint32_t o0[DM];
+    uint32_t sj[DM];
     if (deg > 0) {
       // every block's index and H values in one batch of loads (padding
       // slots: the last block, weight 0)
@@ -367,14 +370,14 @@ __device__ __forceinline__ void mfma_step_rr_dev(
 #pragma unroll
       for (int u = 0; u < DM; u++) {
         mfma_a_of<S>(hr[u], m, u < deg ? S(1) : S(0), a[u]);
-        o0[u] = ((uint32_t)(8 * jj[u] + m.q) * rs + 2u * m.c) * ES;
+        sj[u] = (uint32_t)__builtin_amdgcn_readfirstlane(8 * jj[u]) * rs * ES;
       }
     } else {
 #pragma unroll
       for (int u = 0; u < DM; u++) {
 #pragma unroll
         for (int k = 0; k < 4; k++) a[u][k] = S(0);
-        o0[u] = 0u;
+        sj[u] = 0u;
       }
     }
     const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 2u * m.c) * ES;
@@ -387,8 +390,8 @@ __device__ __forceinline__ void mfma_step_rr_dev(
 #pragma unroll
         for (int v = 0; v < G; v++) {
           if (u0 + v >= DM) continue;  // DM % G != 0: the last group is short
-          x0[v] = IO::ld(bx, o0[u0 + v], so);
-          x1[v] = IO::ld(bx, o0[u0 + v] + half, so);
+          x0[v] = IO::ld(bx, vlo, sj[u0 + v] + so);
+          x1[v] = IO::ld(bx, vlo, sj[u0 + v] + so + half);
         }
 #pragma unroll
         for (int v = 0; v < G; v++)
@@ -464,6 +467,15 @@ __device__ __forceinline__ void mfma_flush_D(const dv4& G, const MfmaLane& m, do
   }
 }
 
+// Neighbour blocks whose loads the row-resident autodiff step issues together
+// before their products: with the A operands in LDS, four fit the registers
+// of 2 waves per SIMD in FP64 on 7 block slots (232 VGPRs + 24 AGPRs), two on
+// 8 slots (four would leave one wave), two in FP32 (3 waves).
+template <typename S, int DM>
+constexpr int mf_rr_group() {
+  return sizeof(S) == 8 && DM == 7 ? 4 : 2;
+}
+
 // ---------------------------------------------------------------------------
 // Row-resident autodiff step for rows of at most DM blocks (same contract as
 // mfma_autodiff_dev; the host uses it when every row has at most DM blocks
@@ -503,12 +515,21 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
   const int dorb = m.c & 7, dpart = m.c >> 3;
   const uint32_t half = 4u * rs * ES;
 
+  // This wave's slice of LDS holds the row's A operands across the tile loop
+  // (lane-major, four values a lane), and a neighbour's offset splits into a
+  // lane constant (VGPR) and the block's wave-uniform part 8 j rs ES (the
+  // buffer load's SGPR offset): the registers go to loads in flight instead.
+  __shared__ S lds_a[WG / WAVE][DM][WAVE][4];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint32_t vlo = (m.q * rs + 2u * m.c) * ES;                    // MFMA operand layout
+  const uint32_t vld = (dorb * rs + (uint32_t)dpart + 2u * m.q) * ES;  // D layout
+  S (&la)[DM][WAVE][4] = lds_a[g];
+
   for (int base = rr.r0; base < rr.r1; base += rr.stride) {
     const int i = __builtin_amdgcn_readfirstlane(base + g);
     if (i >= rr.r1) continue;
     const int p0 = rp[i], deg = rp[i + 1] - p0;
-    S a[DM][4];
-    uint32_t o0[DM], od[DM];
+    uint32_t sj[DM];  // 8 j rs ES of block u's column block j (wave-uniform)
     if (deg > 0) {
       // every block's index and H values in one batch of loads
       int jj[DM];
@@ -521,20 +542,21 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
       }
 #pragma unroll
       for (int u = 0; u < DM; u++) {
-        mfma_a_of<S>(hr[u], m, u < deg ? S(1) : S(0), a[u]);
-        o0[u] = ((uint32_t)(8 * jj[u] + m.q) * rs + 2u * m.c) * ES;
-        od[u] = ((uint32_t)(8 * jj[u] + dorb) * rs + (uint32_t)dpart + 2u * m.q) * ES;
+        S a[4];
+        mfma_a_of<S>(hr[u], m, u < deg ? S(1) : S(0), a);
+#pragma unroll
+        for (int q = 0; q < 4; q++) la[u][lane][q] = a[q];
+        sj[u] = (uint32_t)__builtin_amdgcn_readfirstlane(8 * jj[u]) * rs * ES;
       }
     } else {
 #pragma unroll
       for (int u = 0; u < DM; u++) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) a[u][k] = S(0);
-        o0[u] = od[u] = 0u;
+        for (int q = 0; q < 4; q++) la[u][lane][q] = S(0);
+        sj[u] = 0u;
       }
     }
-    const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 2u * m.c) * ES;
-    const uint32_t fd = ((uint32_t)(8 * i + dorb) * rs + (uint32_t)dpart + 2u * m.q) * ES;
+    const uint32_t si = (uint32_t)(8 * i) * rs * ES;  // the row's own block (wave-uniform)
     dv4 G[DM];
 #pragma unroll
     for (int u = 0; u < DM; u++) G[u] = dv4{0.0, 0.0, 0.0, 0.0};
@@ -544,7 +566,7 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
       S pv[4];
       if constexpr (FUSED_D) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) pv[k] = IO::ld1(ba, fd + 8u * k * ES, so);
+        for (int k = 0; k < 4; k++) pv[k] = IO::ld1(ba, vld + 8u * k * ES, si + so);
       }
 #pragma unroll
       for (int u0 = 0; u0 < DM; u0 += GR) {
@@ -553,20 +575,24 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
 #pragma unroll
         for (int v = 0; v < GR; v++) {
           if (u0 + v >= DM) continue;  // DM % GR != 0: the last group is short
-          xa0[v] = IO::ld(ba, o0[u0 + v], so);
-          xa1[v] = IO::ld(ba, o0[u0 + v] + half, so);
-          xb0[v] = IO::ld(bb, o0[u0 + v], so);
-          xb1[v] = IO::ld(bb, o0[u0 + v] + half, so);
+          const uint32_t sb_ = sj[u0 + v] + so;
+          xa0[v] = IO::ld(ba, vlo, sb_);
+          xa1[v] = IO::ld(ba, vlo, sb_ + half);
+          xb0[v] = IO::ld(bb, vlo, sb_);
+          xb1[v] = IO::ld(bb, vlo, sb_ + half);
           if constexpr (FUSED_D) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) qv[v][k] = IO::ld1(bb, od[u0 + v] + 8u * k * ES, so);
+            for (int k = 0; k < 4; k++) qv[v][k] = IO::ld1(bb, vld + 8u * k * ES, sb_);
           }
         }
 #pragma unroll
         for (int v = 0; v < GR; v++) {
           if (u0 + v >= DM) continue;
-          accA = mfma_block<S>(a[u0 + v], xa0[v], xa1[v], accA);
-          accB = mfma_block<S>(a[u0 + v], xb0[v], xb1[v], accB);
+          S au[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) au[q] = la[u0 + v][lane][q];
+          accA = mfma_block<S>(au, xa0[v], xa1[v], accA);
+          accB = mfma_block<S>(au, xb0[v], xb1[v], accB);
           if constexpr (FUSED_D) {
             V4 gt = V4{S(0), S(0), S(0), S(0)};
 #pragma unroll
@@ -575,9 +601,9 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
           }
         }
       }
-      const C2 ai0 = IO::ld(ba, f0, so), ai1 = IO::ld(ba, f0 + half, so);
-      const C2 a10 = IO::ld(ba1, f0, so), a11 = IO::ld(ba1, f0 + half, so);
-      const C2 b20 = IO::ld(bb2, f0, so), b21 = IO::ld(bb2, f0 + half, so);
+      const C2 ai0 = IO::ld(ba, vlo, si + so), ai1 = IO::ld(ba, vlo, si + so + half);
+      const C2 a10 = IO::ld(ba1, vlo, si + so), a11 = IO::ld(ba1, vlo, si + so + half);
+      const C2 b20 = IO::ld(bb2, vlo, si + so), b21 = IO::ld(bb2, vlo, si + so + half);
       const S hx[4] = {accA.x, accA.y, accA.z, accA.w};
       const S hb[4] = {accB.x, accB.y, accB.z, accB.w};
       const S ai[4] = {ai0.x, ai0.y, ai1.x, ai1.y};
@@ -593,10 +619,10 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
         v = fma(aw, an[e], v);
         bo[e] = v;
       }
-      IO::st(C2{an[0], an[1]}, bao, f0, so);
-      IO::st(C2{an[2], an[3]}, bao, f0 + half, so);
-      IO::st(C2{bo[0], bo[1]}, bbo, f0, so);
-      IO::st(C2{bo[2], bo[3]}, bbo, f0 + half, so);
+      IO::st(C2{an[0], an[1]}, bao, vlo, si + so);
+      IO::st(C2{an[2], an[3]}, bao, vlo, si + so + half);
+      IO::st(C2{bo[0], bo[1]}, bbo, vlo, si + so);
+      IO::st(C2{bo[2], bo[3]}, bbo, vlo, si + so + half);
     }
     if constexpr (FUSED_D) {
 #pragma unroll
@@ -844,8 +870,9 @@ __global__ __launch_bounds__(WG) void k_mfma_autodiff_rr(
     const S* __restrict__ bm2, S* __restrict__ a_out, S* __restrict__ b_out,
     double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
     double sb, double sb2, double dscale, int nt) {
-  mfma_autodiff_rr_dev<S, true, DM, 1>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out,
-                                       dacc, nb, qp, chunk, c4, c2, w, sb, sb2, dscale, nt);
+  mfma_autodiff_rr_dev<S, true, DM, mf_rr_group<S, DM>()>(wg_self(), rp, ci, hv, am, am1, bm1, bm2,
+                                                         a_out, b_out, dacc, nb, qp, chunk, c4, c2,
+                                                         w, sb, sb2, dscale, nt);
 }
 
 }  // namespace fkd
