@@ -6,5 +6,6 @@ include/fastkpm.h).  This package is the Python mirror of the engine interface
 over the C-ABI, plus the synthetic lattice generators used by the tests and the
 benchmark.
 """
-from .engine import EnergyScale, HipEngine, SpMatBsr, bandwidth_probe, device_count  # noqa: F401
+from .engine import (EnergyScale, HipEngine, SpMatBsr, bandwidth_probe, device_count,  # noqa: F401
+                     lattice_order)
 from . import lattice  # noqa: F401

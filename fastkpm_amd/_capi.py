@@ -55,6 +55,7 @@ SIGNATURES = {
     "fkpm_get_stoch_method": (_I, [_P, _IP]),
     "fkpm_get_state": (_I, [_P, _IP, _IP, _IP]),
     "fkpm_get_graph_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "fkpm_lattice_order": (_I, [_I, _I, _IP, _IP, _IP, _IP]),
     "fkpm_moments_noalloc": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _D, _D, _I, _I, _P, _I, _P,
                                   _P]),
     # batched engine

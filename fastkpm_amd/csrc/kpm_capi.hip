@@ -603,9 +603,10 @@ std::map<std::string, std::array<KCfg, NKIND>> g_tune_cache;
 
 std::string tune_key(const fkpm_handle* h) {
   char k[128];
-  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb,
+  snprintf(k, sizeof(k), "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d", h->dtype, h->prec, h->b, h->s, h->nb,
            h->nnzb, h->mfma_on ? 1 : 0,
-           h->max_deg <= 7 ? 2 : h->max_deg <= kRowResidentMax ? 1 : 0, table_wanted(h) ? 1 : 0);
+           h->max_deg <= 7 ? 2 : h->max_deg <= kRowResidentMax ? 1 : 0, table_wanted(h) ? 1 : 0,
+           h->has_order ? 1 : 0);
   return k;
 }
 
@@ -2201,6 +2202,20 @@ int fkpm_get_kernel_sweep(fkpm_handle* h, int kind, int* sweep) {
   if (kind < 0 || kind >= NKIND) return fail(FKPM_E_ARG, "bad kernel kind %d", kind);
   *sweep = h->kc[kind].sweep;
   return 0;
+}
+
+int fkpm_lattice_order(int nb, int nnzb, const int* row_ptr, const int* col_idx, int* order,
+                       int* dims) {
+  if (nb <= 0 || nnzb < 0 || !row_ptr || (nnzb > 0 && !col_idx) || !order || !dims)
+    return fail(FKPM_E_ARG, "lattice_order: bad arguments");
+  CHK(check_pattern("lattice_order", nb, nnzb, row_ptr, nullptr, col_idx));
+  std::vector<int> ord;
+  if (!lattice_order(nb, row_ptr, col_idx, ord)) return 0;
+  int n[3];
+  lattice_dims(nb, row_ptr, col_idx, n);
+  std::copy(ord.begin(), ord.end(), order);
+  std::copy(n, n + 3, dims);
+  return 1;
 }
 
 int fkpm_get_graph_stats(fkpm_handle* h, long long* launches, long long* captures) {

@@ -367,5 +367,21 @@ def bandwidth_probe(eng: "HipEngine", nbytes: int = 1 << 30, reps: int = 10) -> 
     return g.value
 
 
+def lattice_order(H: SpMatBsr):
+    """The lattice row order set_H builds for H's pattern (FKPM_OPT_ROW_ORDER):
+    ((n1, n2, n3), order) when H is a periodic 3-D lattice in natural order,
+    else None.  A host computation; no device needed."""
+    nb = int(H.n_rows)
+    rp = np.ascontiguousarray(H.row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(H.col_idx, dtype=np.int32)
+    order = np.empty(nb, dtype=np.int32)
+    dims = np.zeros(3, dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    rc = lib().fkpm_lattice_order(nb, int(len(ci)), ip(rp), ip(ci), ip(order), ip(dims))
+    if rc < 0:
+        _capi.check(rc, "fkpm_lattice_order")
+    return (tuple(int(d) for d in dims), order) if rc == 1 else None
+
+
 def device_count() -> int:
     return int(lib().fkpm_device_count())

@@ -274,6 +274,16 @@ int fkpm_get_alpha_table(fkpm_handle* h, int* stored);
 /* Step loops replayed as a hipGraph and graphs captured so far (FKPM_OPT_GRAPH). */
 int fkpm_get_graph_stats(fkpm_handle* h, long long* launches, long long* captures);
 
+/* The lattice row order fkpm_set_H would build for a block pattern
+ * (FKPM_OPT_ROW_ORDER), a host computation that needs no device: returns 1
+ * and fills dims[3] = (n1, n2, n3) and order[nb] (the block row at each sweep
+ * position, XCD x's slab at positions [x nb/8, (x+1) nb/8)) when the pattern
+ * is a periodic 3-D lattice in natural order; 0 (order and dims untouched)
+ * otherwise; FKPM_E_ARG for bad arguments.  No counterpart in the reference:
+ * a diagnostic of this library's sweep. */
+int fkpm_lattice_order(int nb, int nnzb, const int* row_ptr, const int* col_idx, int* order,
+                       int* dims);
+
 /* *method = the FKPM_STOCH_* method the last fkpm_stoch_matrix ran (-1 if
  * none yet). */
 int fkpm_get_stoch_method(fkpm_handle* h, int* method);
