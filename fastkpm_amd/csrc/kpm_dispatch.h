@@ -85,11 +85,15 @@ bool find_kset_fr(int B, int L, int KQ, KSet<float>* out);   // float, real
 // packet per lane keeps the register count (and so the occupancy) of the
 // fused autodiff step low at a given pass width; four packets over eight
 // lanes gives it longer loads per lane (the FP32 autodiff step at S = 64).
+// For 2x2 blocks, two packets over a whole wave carry 128 FP64 columns in one
+// pass with the row wave-uniform (scalar row offsets, indices and H) at 3
+// waves per SIMD in the autodiff step, where 32 lanes x 4 packets hold 2 (the
+// tuner takes it for C3's FP64 forward step, with the lattice row order).
 #define FKD_FOR_EACH_GEOMETRY(X)                                                       \
   X(1, 4, 1) X(1, 4, 2) X(1, 8, 2) X(1, 8, 4) X(1, 16, 1) X(1, 16, 2) X(1, 16, 4)      \
   X(1, 32, 1) X(1, 32, 4) X(1, 64, 1) X(1, 64, 4)                                      \
   X(2, 4, 1) X(2, 4, 2) X(2, 8, 2) X(2, 8, 4) X(2, 16, 1) X(2, 16, 2) X(2, 16, 4)      \
-  X(2, 32, 1) X(2, 32, 4) X(2, 64, 1) X(2, 64, 4)                                      \
+  X(2, 32, 1) X(2, 32, 4) X(2, 64, 1) X(2, 64, 2) X(2, 64, 4)                          \
   X(4, 4, 1) X(4, 4, 2) X(4, 8, 2) X(4, 16, 1) X(4, 16, 2) X(4, 32, 1) X(4, 32, 2)     \
   X(4, 64, 1) X(4, 64, 2)                                                              \
   X(8, 4, 1) X(8, 8, 1) X(8, 16, 1) X(8, 32, 1) X(8, 64, 1)

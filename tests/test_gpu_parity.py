@@ -399,7 +399,7 @@ def test_every_compiled_geometry(oracle):
     mu_o = orc.moments(M)
     d_o = orc.autodiff_matrix(c, H)
     s_o = orc.stoch_matrix(c, H)
-    geos = [(4, 1), (4, 2), (8, 2), (16, 2), (16, 4), (32, 4), (64, 1), (64, 4)]
+    geos = [(4, 1), (4, 2), (8, 2), (16, 2), (16, 4), (32, 4), (64, 1), (64, 2), (64, 4)]
     for L, KQ in geos:
         for maxwg in (0, 7):
             eng = _engine()
@@ -466,7 +466,7 @@ def test_lattice_row_order(oracle, prec):
     c = fermi_coeffs(oracle, M, es.lo, es.hi)
     mu_o, d_o, s_o = orc.moments(M), orc.autodiff_matrix(c, H), orc.stoch_matrix(c, H)
     tol, tol_d = (FP64_TOL, FP64_TOL) if prec == "fp64" else (2e-5, 1e-4)
-    for L, KQ in [(64, 1), (16, 2), (4, 1)]:
+    for L, KQ in [(64, 1), (64, 2), (16, 2), (4, 1)]:
         out = {}
         for order in (0, 1):
             for store, method in ((0, "clenshaw"), (2, "forward")):
