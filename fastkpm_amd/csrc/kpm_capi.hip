@@ -1627,28 +1627,31 @@ bool lattice_dims(int nb, const int* rp, const int* ci, int n[3]) {
   return n[0] >= 8 && n[1] >= 8 && n[2] >= 8;
 }
 
-// Sweep order for an n1 x n2 x n3 lattice: one x-slab of n1 / 8 sites per XCD
-// (positions [x nb/8, (x+1) nb/8) of the sweep), each walked in y-tiles of
-// ty rows (about 128 sites a plane of the tile) and, inside a tile, plane by
-// plane along z.  A row's +-z neighbours are then one tile-plane away in the
-// sweep instead of a whole slab-plane (on 64^3: 128 instead of 512 sites), so
-// the neighbour rows of both vectors an autodiff step gathers stay in the
-// XCD's L2.  Requires n1 % 8 == 0.
+// Sweep order for an n1 x n2 x n3 lattice: one x-slab per XCD (x in
+// [floor(k n1 / 8), floor((k+1) n1 / 8)) for XCD k: widths n1/8 rounded down
+// or up, about one eighth of the sweep each, the same split into eighths that
+// the per-XCD row ranges of the sweep make), each walked in y-tiles of ty rows
+// (about 128 sites a plane of the tile) and, inside a tile, plane by plane
+// along z.  A row's +-z neighbours are then one tile-plane away in the sweep
+// instead of a whole slab-plane (on 64^3: 128 instead of 512 sites), so the
+// neighbour rows of both vectors an autodiff step gathers stay in the XCD's
+// L2.
 bool lattice_order(int nb, const int* rp, const int* ci, std::vector<int>& order) {
   int n[3];
-  if (!lattice_dims(nb, rp, ci, n) || n[0] % 8 != 0) return false;
-  const int w = n[0] / 8;
+  if (!lattice_dims(nb, rp, ci, n)) return false;
+  const int wmax = (n[0] + 7) / 8;
   int ty = 1;
   for (int t = 1; t <= n[1]; t++)
-    if (n[1] % t == 0 && w * t <= 128) ty = t;
+    if (n[1] % t == 0 && wmax * t <= 128) ty = t;
   order.clear();
   order.reserve(nb);
-  for (int x = 0; x < 8; x++)
+  for (int k = 0; k < 8; k++) {
+    const int x0 = (int)((long long)k * n[0] / 8), x1 = (int)((long long)(k + 1) * n[0] / 8);
     for (int y0 = 0; y0 < n[1]; y0 += ty)
       for (int z = 0; z < n[2]; z++)
         for (int yo = 0; yo < ty; yo++)
-          for (int xo = 0; xo < w; xo++)
-            order.push_back(x * w + xo + n[0] * ((y0 + yo) + n[1] * z));
+          for (int x = x0; x < x1; x++) order.push_back(x + n[0] * ((y0 + yo) + n[1] * z));
+  }
   return (int)order.size() == nb;
 }
 

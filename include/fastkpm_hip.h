@@ -211,9 +211,9 @@ int fkpm_moments_noalloc(int device, int prec, int nb, int b, int nnzb, const in
                                    fkpm_dev_open sets the handle's default */
 #define FKPM_OPT_ROW_ORDER 9    /* value 1: when H's pattern is a periodic 3-D lattice in
                                    natural order (block row x + n1 (y + n2 z), every side at
-                                   least 8, n1 a multiple of 8; recognised at fkpm_set_H from
-                                   the block offsets), the vector-unit step kernels visit the
-                                   rows slab by slab (one x-slab per XCD) and, in a slab, in
+                                   least 8; recognised at fkpm_set_H from the block
+                                   offsets), the vector-unit step kernels visit the rows
+                                   slab by slab (one x-slab of about n1/8 sites per XCD), in
                                    y-tiles along z, so the next plane's rows are still in the
                                    XCD's L2; results per row are the same as in row order, the
                                    moment sums add in another order; 0: row order; -1
@@ -277,7 +277,8 @@ int fkpm_get_graph_stats(fkpm_handle* h, long long* launches, long long* capture
 /* The lattice row order fkpm_set_H would build for a block pattern
  * (FKPM_OPT_ROW_ORDER), a host computation that needs no device: returns 1
  * and fills dims[3] = (n1, n2, n3) and order[nb] (the block row at each sweep
- * position, XCD x's slab at positions [x nb/8, (x+1) nb/8)) when the pattern
+ * position: XCD k's slab, x in [floor(k n1/8), floor((k+1) n1/8)), after
+ * those of XCDs 0 .. k-1) when the pattern
  * is a periodic 3-D lattice in natural order; 0 (order and dims untouched)
  * otherwise; FKPM_E_ARG for bad arguments.  No counterpart in the reference:
  * a diagnostic of this library's sweep. */

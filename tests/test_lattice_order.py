@@ -1,8 +1,8 @@
 """The lattice row order (FKPM_OPT_ROW_ORDER) that set_H builds for a periodic
 3-D lattice, computed on the host through fkpm_lattice_order: a permutation of
-the block rows, one x-slab of n1/8 sites per eighth of the sweep, each walked
-in y-tiles plane by plane along z; no order for 2-D lattices, thin lattices or
-patterns with extra bonds."""
+the block rows, one x-slab of about n1/8 sites per XCD, each walked in y-tiles
+plane by plane along z; no order for 2-D lattices, thin lattices or patterns
+with extra bonds."""
 from __future__ import annotations
 
 import numpy as np
@@ -17,7 +17,7 @@ def _coords(order, n1, n2):
 
 
 @pytest.mark.parametrize("shape,b", [((64, 64, 64), 1), ((16, 8, 24), 2), ((24, 16, 16), 2),
-                                     ((8, 8, 8), 8)])
+                                     ((8, 8, 8), 8), ((12, 12, 12), 2), ((76, 10, 9), 2)])
 def test_order_of_a_cubic_lattice(shape, b):
     from fastkpm_amd import lattice, lattice_order
     H = lattice.lattice(shape, b=b, seed=1, orbital_blocks=(b == 8))
@@ -28,14 +28,19 @@ def test_order_of_a_cubic_lattice(shape, b):
     nb = H.n_rows
     assert np.array_equal(np.sort(order), np.arange(nb))
     n1, n2, n3 = shape
-    w = n1 // 8
     x, y, z = _coords(order, n1, n2)
-    # XCD slab: positions [k nb/8, (k+1) nb/8) hold x in [k w, (k+1) w)
-    assert np.array_equal(x // w, np.arange(nb) // (nb // 8))
+    # XCD slabs: x in [floor(k n1 / 8), floor((k+1) n1 / 8)), in order of k
+    edges = [k * n1 // 8 for k in range(9)]
+    slab = np.searchsorted(edges, x, side="right") - 1
+    assert np.all(np.diff(slab) >= 0) and slab[0] == 0 and slab[-1] == 7
+    wmax = -(-n1 // 8)
+    ty = max(t for t in range(1, n2 + 1) if n2 % t == 0 and wmax * t <= 128)
     # inside a slab: the y-tile, then z, then y in the tile, then x
-    ty = max(t for t in range(1, n2 + 1) if n2 % t == 0 and w * t <= 128)
-    key = (((x // w) * (n2 // ty) + y // ty) * n3 + z) * ty * w + (y % ty) * w + x % w
-    assert np.array_equal(key, np.arange(nb))
+    w = np.diff(edges)[slab]
+    x0 = np.asarray(edges)[slab]
+    key = ((y // ty) * n3 + z) * ty * w + (y % ty) * w + (x - x0)
+    start = np.asarray([0] + list(np.cumsum(np.diff(edges) * n2 * n3)))[slab]
+    assert np.array_equal(start + key, np.arange(nb))
     # a row's +z neighbour is one tile-plane (ty w sites) on in the sweep,
     # except across the periodic boundary
     pos = np.empty(nb, dtype=np.int64)
@@ -43,14 +48,14 @@ def test_order_of_a_cubic_lattice(shape, b):
     site = np.arange(nb)
     up = (site + n1 * n2) % nb
     inner = (site // (n1 * n2)) < n3 - 1
-    assert np.all(pos[up[inner]] - pos[site[inner]] == ty * w)
+    ws = np.diff(edges)[np.searchsorted(edges, site % n1, side="right") - 1]
+    assert np.all(pos[up[inner]] - pos[site[inner]] == ty * ws[inner])
 
 
 def test_no_order_for_other_patterns():
     from fastkpm_amd import lattice, lattice_order
     assert lattice_order(lattice.lattice((256, 256), b=2, seed=1)) is None  # 2-D
     assert lattice_order(lattice.lattice((4, 64, 64), b=2, seed=1)) is None  # n1 < 8
-    assert lattice_order(lattice.lattice((12, 12, 12), b=2, seed=1)) is None  # n1 % 8 != 0
     H = lattice.lattice((4, 4, 4), b=2, seed=1)
     assert lattice_order(H) is None  # fewer than 512 block rows
     shape = (16, 16, 16)
