@@ -60,7 +60,8 @@ struct SlotDev {
   double* partial;  // [M/2][nwg][2] per-workgroup moment dots
   double* psum;     // [M/2][2]
   const double* c;  // M coefficients
-  int nb, qp, chunk, nwg, M, pad;
+  const int* rord;  // lattice row order: {row, first block, end block} per sweep position
+  int nb, qp, chunk, nwg, M, nt;  // nt: NT_ORDER when the step kernels sweep rord
   double inv_mag, diag, sb;
 };
 
@@ -108,8 +109,9 @@ __global__ __launch_bounds__(WG) void k_batch_fwd(const SlotDev* __restrict__ sd
     mfma_step_dev<S>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp, d.chunk, cx, cz, part,
                      0, nors);
   else
-    step_moments_dev<S, true, B, L, KQ>(wid, d.rp, d.ci, (const S*)d.hv, x, z, y, d.nb, d.qp,
-                                        d.chunk, cx, cz, part, 0, nors);
+    step_moments_dev<S, true, B, L, KQ>(wid, (d.nt & NT_ORDER) ? d.rord : d.rp, d.ci,
+                                        (const S*)d.hv, x, z, y, d.nb, d.qp, d.chunk, cx, cz, part,
+                                        d.nt, nors);
 }
 
 // Backward (autodiff) step of every slot of the class in its backward phase;
@@ -146,9 +148,10 @@ __global__ __launch_bounds__(WG) void k_batch_bwd(const SlotDev* __restrict__ sd
     mfma_autodiff_dev<S, true>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2, aout, bout,
                                d.dacc, d.nb, d.qp, d.chunk, c4, c2, w, d.sb, sb2, 2.0, 0);
   else
-    step_autodiff_dev<S, true, B, L, KQ, true>(wid, d.rp, d.ci, (const S*)d.hv, am, am1, bm1, bm2,
-                                               aout, bout, d.dacc, d.nb, d.qp, d.chunk, c4, c2, w,
-                                               d.sb, sb2, 2.0, 0);
+    step_autodiff_dev<S, true, B, L, KQ, true>(wid, (d.nt & NT_ORDER) ? d.rord : d.rp, d.ci,
+                                               (const S*)d.hv, am, am1, bm1, bm2, aout, bout,
+                                               d.dacc, d.nb, d.qp, d.chunk, c4, c2, w, d.sb, sb2,
+                                               2.0, d.nt);
 }
 
 // Auxiliary work of a tick: zero D (tick 0), D = diag R R^+ (tick M/2),
@@ -404,6 +407,7 @@ struct Request {
   int nb = 0, b = 0, nnzb = 0, n = 0, s = 0, M = 0;
   double lo = 0, hi = 0;
   std::vector<int> rp, ci, rowblk, tpos;
+  std::vector<int> rdesc;  // lattice row order descriptors (large 3-D lattices), else empty
   int max_deg = 0;  // most blocks in a row
   std::vector<double> val, R, c;
   bool compact = false;  // R given as (rcol, rval) per orbital row (R holds rval)
@@ -428,7 +432,7 @@ struct Layout {
 };
 
 enum Buf { B_RP, B_CI, B_RB, B_TP, B_HV, B_HRAW, B_R, B_A0, B_A1, B_A2, B_B0, B_B1, B_B2, B_RRAW,
-           B_DACC, B_DOUT, B_PART, B_PSUM, B_C, B_COUNT };
+           B_DACC, B_DOUT, B_PART, B_PSUM, B_C, B_RD, B_COUNT };
 
 // mf: the matrix-core form a b = 8 request takes (1 general, 2 row-resident on
 // 8 slots, 3 on 7); vector units (0) for b = 2.
@@ -471,14 +475,20 @@ void plan(bool single, const Request& r, Layout* L) {
   L->qp = ((need + L->W - 1) / L->W) * L->W;
   L->sp = L->qp * ne;
   // workgroups per slot: one per 4 block rows up to 2048 (fixed by the
-  // shape); they sweep the slot's rows as one front (chunk -1): a slot's
-  // first workgroup sits at any offset of the launch, so per-XCD regions
-  // would not line up with the XCDs
+  // shape); they sweep the slot's rows as one front (chunk -1), or, with the
+  // lattice row order, the sweep positions one range per XCD (chunk > 0).  A
+  // slot's first workgroup sits at any offset of the launch, so workgroup k
+  // of the slot runs on XCD (offset + k) mod 8: the ranges keep to one XCD
+  // each, only which XCD takes which range shifts.
   int nwg = std::min(std::max((r.nb + 3) / 4, 1), 2048);
   int chunk = (r.nb + nwg - 1) / nwg;
+  if (!r.rdesc.empty()) {
+    const int gwg = (fkd::WG / fkd::WAVE) * (fkd::WAVE / classes(single)[L->cls].L);
+    chunk = (chunk + gwg - 1) / gwg * gwg;
+  }
   nwg = (r.nb + chunk - 1) / chunk;
   L->nwg = nwg;
-  L->chunk = -1;
+  L->chunk = r.rdesc.empty() ? -1 : chunk;
   const size_t vec = (size_t)r.n * L->sp * 2 * es;
   const size_t dbl = (size_t)r.nnzb * r.b * r.b * 2 * sizeof(double);
   const size_t h = r.M / 2;
@@ -497,6 +507,7 @@ void plan(bool single, const Request& r, Layout* L) {
   len[B_PART] = h * nwg * 2 * sizeof(double);
   len[B_PSUM] = h * 2 * sizeof(double);
   len[B_C] = sizeof(double) * r.M;
+  len[B_RD] = sizeof(int) * r.rdesc.size();
   L->nbuf = B_COUNT;
   L->total = 0;
   for (int k = 0; k < B_COUNT; k++) {
@@ -624,6 +635,7 @@ void admit_one(fkpm_batch* B, Slot& sl) {
     h2d(P(B_RRAW), r.R, L.len[B_RRAW], st);
   }
   h2d(P(B_C), r.c, L.len[B_C], st);
+  h2d(P(B_RD), r.rdesc, L.len[B_RD], st);
   const double avg = (r.hi + r.lo) / 2, mag = (r.hi - r.lo) / 2;
   if (fkpm_internal::scale_H_cx(st, (const double*)P(B_HRAW), P(B_HV), B->single,
                                 (const int*)P(B_RB), (const int*)P(B_CI), r.nnzb, r.b, mag,
@@ -668,6 +680,8 @@ void admit_one(fkpm_batch* B, Slot& sl) {
   d.partial = (double*)P(B_PART);
   d.psum = (double*)P(B_PSUM);
   d.c = (const double*)P(B_C);
+  d.rord = r.rdesc.empty() ? nullptr : (const int*)P(B_RD);
+  d.nt = r.rdesc.empty() ? 0 : fkd::NT_ORDER;
   d.nb = r.nb;
   d.qp = L.qp;
   d.chunk = L.chunk;
@@ -966,6 +980,11 @@ int fkpm_batch_footprint(int prec, int nb, int b, int nnzb, int s, int M, size_t
   Layout L;
   plan(prec == FKPM_PREC_FP32, r, &L);
   *bytes = L.total;
+  if (b == 2 && nb >= 4096) {  // a 3-D lattice pattern also holds its row order
+    r.rdesc.assign(3 * (size_t)nb, 0);
+    plan(prec == FKPM_PREC_FP32, r, &L);
+    *bytes = std::max(*bytes, L.total);
+  }
   return 0;
 }
 
@@ -1129,6 +1148,20 @@ static int submit_common(fkpm_batch* B, int nb, int b, int nnzb, const int* row_
     r->R.assign((const double*)rval, (const double*)rval + (size_t)n * 2);
   }
   r->c.assign(c, c + M);
+  // large 3-D lattices with 2x2 blocks (the vector-unit classes) sweep their
+  // rows in the lattice row order, as the single-request engine's tuner
+  // chooses for them
+  if (b == 2 && nb >= 4096) {
+    std::vector<int> ord;
+    if (fkpm_internal::lattice_row_order(nb, r->rp.data(), r->ci.data(), ord)) {
+      r->rdesc.resize(3 * (size_t)nb);
+      for (int k = 0; k < nb; k++) {
+        r->rdesc[3 * (size_t)k] = ord[k];
+        r->rdesc[3 * (size_t)k + 1] = r->rp[ord[k]];
+        r->rdesc[3 * (size_t)k + 2] = r->rp[ord[k] + 1];
+      }
+    }
+  }
   Layout L;
   plan(B->single, *r, &L);
   if (L.cls < 0) return berr(FKPM_E_UNSUP, "batch: no kernel class for block %d", b);

@@ -2397,4 +2397,8 @@ int pack_R_cx(hipStream_t st, const double* raw, void* dst, bool single, int n, 
 
 int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 
+bool lattice_row_order(int nb, const int* rp, const int* ci, std::vector<int>& order) {
+  return lattice_order(nb, rp, ci, order);
+}
+
 }  // namespace fkpm_internal

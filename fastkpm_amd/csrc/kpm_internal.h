@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace fkpm_internal {
 
 // Hs = (H - shift I) / mag from complex-double host-layout values `raw` (device)
@@ -16,5 +18,10 @@ int pack_R_cx(hipStream_t st, const double* raw, void* dst, bool single, int n, 
 
 // Record an error message for fkpm_last_error() and return `code`.
 int set_error(int code, const char* msg);
+
+// The lattice row order the single-request engine builds for a periodic 3-D
+// lattice pattern in natural order (FKPM_OPT_ROW_ORDER): the block row at each
+// sweep position, XCD k's slab in the k-th eighth; false for other patterns.
+bool lattice_row_order(int nb, const int* rp, const int* ci, std::vector<int>& order);
 
 }  // namespace fkpm_internal

@@ -319,7 +319,9 @@ typedef struct fkpm_batch fkpm_batch;
 #define FKPM_BATCH_OPT_GRAPH 1 /* value 1 (default): capture runs of ticks in hipGraphs */
 #define FKPM_BATCH_OPT_TICKS 2 /* value: most ticks per replay between admissions (16) */
 
-/* HBM bytes one request occupies while admitted (its arena footprint). */
+/* HBM bytes one request occupies while admitted (its arena footprint; for
+ * 2x2 blocks and nb >= 4096 with room for the lattice row order a 3-D
+ * lattice pattern takes, so an upper bound for other patterns). */
 int fkpm_batch_footprint(int prec, int nb, int b, int nnzb, int s, int M, size_t* bytes);
 
 /* Open a batched engine on `device` with an arena of `hbm_bytes` and at most

@@ -162,6 +162,38 @@ def test_batch_seeded_equals_host_probing(shape, b, per):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_lattice_row_order(oracle, prec):
+    """A 3-D lattice request with 4096 block rows takes the lattice row order
+    (its slot sweeps the order's positions one range per XCD), in one class
+    launch with a 2-D request that keeps row order: both against the oracle,
+    and the ordered request bitwise the same alone."""
+    from fastkpm_amd import host, lattice, lattice_order
+    from fastkpm_amd.serving import BatchEngine, gershgorin
+    reqs = []
+    for shape, per, seed in (((16, 16, 16), (2, 2, 2), 51), ((64, 64), (2, 4), 52)):
+        H = lattice.lattice(shape, b=2, J=1.0, W=2.0, seed=seed)
+        es = gershgorin(H)
+        M = 20
+        c = fermi_coeffs(oracle, M, es.lo, es.hi, kT=0.05)
+        R = seeded_R(oracle, lattice.colouring(shape, per, 2), seed + 100)
+        reqs.append((H, es, c, R))
+    assert lattice_order(reqs[0][0]) is not None and lattice_order(reqs[1][0]) is None
+    assert reqs[0][3].shape[1] == reqs[1][3].shape[1] == 16  # the same class
+    tol_mu, tol_d = (FP64_TOL, FP64_TOL) if prec == "fp64" else (2e-5, 1e-4)
+    eng = BatchEngine(prec, hbm_bytes=1 << 30, max_slots=4)
+    ids = [eng.submit(H, es, c, R=R) for H, es, c, R in reqs]
+    res = [eng.wait(rid) for rid in ids]
+    for (H, es, c, R), r in zip(reqs, res):
+        mu_o, D_o = _oracle_solve(oracle, H, es, c, R)
+        assert rel_err(r.mu, mu_o) < tol_mu
+        assert rel_err(r.D, D_o) < tol_d
+    H, es, c, R = reqs[0]
+    alone = eng.wait(eng.submit(H, es, c, R=R))
+    assert np.array_equal(alone.mu, res[0].mu) and np.array_equal(alone.D, res[0].D)
+    eng.close()
+
+
 def _dense_rows_b8(nb=20, extra=6, seed=3):
     """Hermitian 8x8-block matrix with rows of up to 2*extra+1 blocks: more
     than the row-resident matrix-core kernels take."""
