@@ -61,6 +61,14 @@ struct KSet {
   void (*mfma_ad_rr7)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
                       S*, S*, double*, int, int, int, double, double, double, double, double,
                       double, int) = nullptr;
+  // FP32 only: the row-resident autodiff step on pairs of column tiles (the
+  // padded width a multiple of 16 packets), 8 and 7 block slots
+  void (*mfma_ad_rr2)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
+                      S*, S*, double*, int, int, int, double, double, double, double, double,
+                      double, int) = nullptr;
+  void (*mfma_ad_rr2_7)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
+                        S*, S*, double*, int, int, int, double, double, double, double, double,
+                        double, int) = nullptr;
   int B = 0, L = 0, KQ = 0;
 };
 

@@ -17,7 +17,8 @@
 // is the 16x16 real product G = P Q^T over the S columns (P, Q = [re; im] of
 // the two 8 x S strips), also on the matrix cores; D follows from
 //   D_re = G[a][b] + G[8+a][8+b],  D_im = G[8+a][b] - G[a][8+b].
-// G accumulates per tile in the storage type and across tiles in FP64.
+// G accumulates per tile (per pair of tiles in the FP32 pair-of-tiles
+// autodiff step) in the storage type and across tiles in FP64.
 //
 // MFMA 16x16x4 lane layout (64 lanes, l = lane, register i = 0..3):
 //   A (16 x 4): lane l holds A[l % 16][l / 16]
@@ -285,6 +286,12 @@ __device__ void fkd_raw_st_f32x2(bf32x2 v, __amdgpu_buffer_rsrc_t rsrc, int voff
                                  int aux) __asm("llvm.amdgcn.raw.ptr.buffer.store.v2f32");
 __device__ void fkd_raw_st_f64x2(bf64x2 v, __amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
                                  int aux) __asm("llvm.amdgcn.raw.ptr.buffer.store.v2f64");
+
+typedef float bf32x4 __attribute__((ext_vector_type(4)));
+__device__ bf32x4 fkd_raw_ld_f32x4(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ void fkd_raw_st_f32x4(bf32x4 v, __amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                 int aux) __asm("llvm.amdgcn.raw.ptr.buffer.store.v4f32");
 
 __device__ float fkd_raw_ld_f32(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
                                int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.f32");
@@ -634,6 +641,179 @@ __device__ __forceinline__ void mfma_autodiff_rr_dev(
   }
 }
 
+// FP32 row-resident autodiff step on pairs of column tiles (same contract
+// and results as mfma_autodiff_rr_dev<float, true>; qp * 4 a multiple of 64,
+// i.e. an even number of 16-column tiles).  A lane's two columns of a pair,
+// 2c and 2c + 1 of orbital q (and q + 4), are four adjacent floats: one
+// 16-byte load feeds the even and the odd MFMA column sets, so each round trip
+// of a neighbour group's loads serves 32 columns instead of 16.  The D
+// operands pair columns 2n and 2n + 1 (n = 4k + q, k = 0..3), each lane's
+// part (re or im) of both.
+// Neighbour blocks whose loads the pair-of-tiles step issues together: two
+// (24 loads) keep three waves per SIMD; four leave two (C4 FP32: 1561 against
+// 1654 us per step).
+constexpr int MF_RR2_GROUP = 2;
+
+template <int DM, int GR>
+__device__ __forceinline__ void mfma_autodiff_rr2_dev(
+    WgId wid, const int* __restrict__ rp, const int* __restrict__ ci,
+    const float* __restrict__ hv, const float* __restrict__ am, const float* __restrict__ am1,
+    const float* __restrict__ bm1, const float* __restrict__ bm2, float* __restrict__ a_out,
+    float* __restrict__ b_out, double* __restrict__ dacc, int nb, int qp, int chunk, double c4,
+    double c2, double w, double sb, double sb2, double dscale) {
+  using S = float;
+  using C2 = Mfma<float>::c2;
+  using V4 = Mfma<float>::v4;
+  using MF = Mfma<float>;
+  using IO = BufC2<float>;
+  constexpr int NS = Vec16<S>::N;
+  constexpr uint32_t ES = sizeof(S);
+  const MfmaLane m = mfma_lane<S>();
+  const int g = threadIdx.x / WAVE;
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
+  const uint32_t rs = (uint32_t)qp * NS;
+  const int npair = (int)(rs / 64);
+  const uint32_t vbytes = 8u * (uint32_t)nb * rs * ES;
+  const bool vb1 = (bm1 == nullptr);
+  const double hb1 = vb1 ? sb : 1.0;
+  const S* xbp = vb1 ? am : bm1;  // gathered beta_{m+1} (or alpha_m, scaled by sb)
+  const __amdgpu_buffer_rsrc_t ba = vec_rsrc(am, vbytes), bb = vec_rsrc(xbp, vbytes);
+  const __amdgpu_buffer_rsrc_t ba1 = vec_rsrc(am1, vbytes);
+  const __amdgpu_buffer_rsrc_t bb2 = vec_rsrc(bm2 != nullptr ? bm2 : am1, vbytes);
+  const __amdgpu_buffer_rsrc_t bao = vec_rsrc(a_out, vbytes), bbo = vec_rsrc(b_out, vbytes);
+  const S a2 = 2.0f, ab2 = (S)(2.0 * hb1), ac4 = (S)c4, ac2 = (S)c2, aw = (S)w;
+  const S asb2 = bm2 != nullptr ? 1.0f : (S)sb2;  // beta_{m+2} = asb2 * (bm2 or alpha_{m+1})
+  const double fD = dscale * hb1;
+  const int dorb = m.c & 7, dpart = m.c >> 3;
+  const uint32_t half = 4u * rs * ES;
+
+  __shared__ S lds_a[WG / WAVE][DM][WAVE][4];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint32_t vlo = (m.q * rs + 4u * m.c) * ES;  // columns 2c, 2c + 1 of orbital q
+  // D operand of column 2n + h, n = 4k + q: scalar 4n + 2h + dpart of orbital dorb
+  const uint32_t vld = (dorb * rs + 4u * m.q + (uint32_t)dpart) * ES;
+  S (&la)[DM][WAVE][4] = lds_a[g];
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    const int i = __builtin_amdgcn_readfirstlane(base + g);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], deg = rp[i + 1] - p0;
+    uint32_t sj[DM];
+    if (deg > 0) {
+      int jj[DM];
+      HRaw<S> hr[DM];
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        const int pu = p0 + min(u, deg - 1);
+        jj[u] = ci[pu];
+        hr[u] = mfma_hraw<S>(hv + (size_t)pu * 128, m);
+      }
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        S a[4];
+        mfma_a_of<S>(hr[u], m, u < deg ? S(1) : S(0), a);
+#pragma unroll
+        for (int q = 0; q < 4; q++) la[u][lane][q] = a[q];
+        sj[u] = (uint32_t)__builtin_amdgcn_readfirstlane(8 * jj[u]) * rs * ES;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) la[u][lane][q] = S(0);
+        sj[u] = 0u;
+      }
+    }
+    const uint32_t si = (uint32_t)(8 * i) * rs * ES;
+    dv4 G[DM];
+#pragma unroll
+    for (int u = 0; u < DM; u++) G[u] = dv4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < npair; t++) {
+      const uint32_t so = (uint32_t)t * 64u * ES;
+      V4 accAe = V4{0.f, 0.f, 0.f, 0.f}, accAo = accAe, accBe = accAe, accBo = accAe;
+      S pv[4][2];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) pv[k][h] = IO::ld1(ba, vld + (16u * k + 2u * h) * ES, si + so);
+#pragma unroll
+      for (int u0 = 0; u0 < DM; u0 += GR) {
+        bf32x4 xa0[GR], xa1[GR], xb0[GR], xb1[GR];
+        S qv[GR][4][2];
+#pragma unroll
+        for (int v = 0; v < GR; v++) {
+          if (u0 + v >= DM) continue;  // DM % GR != 0: the last group is short
+          const uint32_t sb_ = sj[u0 + v] + so;
+          xa0[v] = fkd_raw_ld_f32x4(ba, (int)vlo, (int)sb_, 0);
+          xa1[v] = fkd_raw_ld_f32x4(ba, (int)vlo, (int)(sb_ + half), 0);
+          xb0[v] = fkd_raw_ld_f32x4(bb, (int)vlo, (int)sb_, 0);
+          xb1[v] = fkd_raw_ld_f32x4(bb, (int)vlo, (int)(sb_ + half), 0);
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) qv[v][k][h] = IO::ld1(bb, vld + (16u * k + 2u * h) * ES, sb_);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // the group's loads all issued before its products
+#pragma unroll
+        for (int v = 0; v < GR; v++) {
+          if (u0 + v >= DM) continue;
+          S au[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) au[q] = la[u0 + v][lane][q];
+          accAe = mfma_block<S>(au, C2{xa0[v].x, xa0[v].y}, C2{xa1[v].x, xa1[v].y}, accAe);
+          accAo = mfma_block<S>(au, C2{xa0[v].z, xa0[v].w}, C2{xa1[v].z, xa1[v].w}, accAo);
+          accBe = mfma_block<S>(au, C2{xb0[v].x, xb0[v].y}, C2{xb1[v].x, xb1[v].y}, accBe);
+          accBo = mfma_block<S>(au, C2{xb0[v].z, xb0[v].w}, C2{xb1[v].z, xb1[v].w}, accBo);
+          V4 gt = V4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) gt = MF::mac(pv[k][h], qv[v][k][h], gt);
+          G[u0 + v] += dv4{(double)gt.x, (double)gt.y, (double)gt.z, (double)gt.w};
+        }
+        // one group's operands live at a time (the scheduler would otherwise
+        // hoist every group's loads and leave one wave per SIMD)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      const bf32x4 ai0 = fkd_raw_ld_f32x4(ba, (int)vlo, (int)(si + so), 0);
+      const bf32x4 ai1 = fkd_raw_ld_f32x4(ba, (int)vlo, (int)(si + so + half), 0);
+      const bf32x4 a10 = fkd_raw_ld_f32x4(ba1, (int)vlo, (int)(si + so), 0);
+      const bf32x4 a11 = fkd_raw_ld_f32x4(ba1, (int)vlo, (int)(si + so + half), 0);
+      const bf32x4 b20 = fkd_raw_ld_f32x4(bb2, (int)vlo, (int)(si + so), 0);
+      const bf32x4 b21 = fkd_raw_ld_f32x4(bb2, (int)vlo, (int)(si + so + half), 0);
+      // column 2c (h = 0) and 2c + 1 (h = 1): (re, im) of orbital q, then q + 4
+      const S hx[2][4] = {{accAe.x, accAe.y, accAe.z, accAe.w}, {accAo.x, accAo.y, accAo.z, accAo.w}};
+      const S hb[2][4] = {{accBe.x, accBe.y, accBe.z, accBe.w}, {accBo.x, accBo.y, accBo.z, accBo.w}};
+      const S ai[2][4] = {{ai0.x, ai0.y, ai1.x, ai1.y}, {ai0.z, ai0.w, ai1.z, ai1.w}};
+      const S a1[2][4] = {{a10.x, a10.y, a11.x, a11.y}, {a10.z, a10.w, a11.z, a11.w}};
+      const S b2[2][4] = {{b20.x, b20.y, b21.x, b21.y}, {b20.z, b20.w, b21.z, b21.w}};
+      S an[2][4], bo[2][4];
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          an[h][e] = fma(a2, hx[h][e], -a1[h][e]);
+          S v = fma(ab2, hb[h][e], -(asb2 * b2[h][e]));
+          v = fma(ac4, ai[h][e], v);
+          v = fma(ac2, a1[h][e], v);
+          v = fma(aw, an[h][e], v);
+          bo[h][e] = v;
+        }
+      fkd_raw_st_f32x4(bf32x4{an[0][0], an[0][1], an[1][0], an[1][1]}, bao, (int)vlo, (int)(si + so), 0);
+      fkd_raw_st_f32x4(bf32x4{an[0][2], an[0][3], an[1][2], an[1][3]}, bao, (int)vlo,
+                       (int)(si + so + half), 0);
+      fkd_raw_st_f32x4(bf32x4{bo[0][0], bo[0][1], bo[1][0], bo[1][1]}, bbo, (int)vlo, (int)(si + so), 0);
+      fkd_raw_st_f32x4(bf32x4{bo[0][2], bo[0][3], bo[1][2], bo[1][3]}, bbo, (int)vlo,
+                       (int)(si + so + half), 0);
+    }
+#pragma unroll
+    for (int u = 0; u < DM; u++) {
+      if (u >= deg) break;
+      mfma_flush_D<S>(G[u], m, fD, dacc + (size_t)(p0 + u) * 128);
+    }
+  }
+}
+
 // Neighbour blocks of one pass of the autodiff kernel: the FP64 G
 // accumulators of the fused outer product are held per block of the pass.
 constexpr int MF_DMAX = 8;
@@ -873,6 +1053,22 @@ __global__ __launch_bounds__(WG) void k_mfma_autodiff_rr(
   mfma_autodiff_rr_dev<S, true, DM, mf_rr_group<S, DM>()>(wg_self(), rp, ci, hv, am, am1, bm1, bm2,
                                                          a_out, b_out, dacc, nb, qp, chunk, c4, c2,
                                                          w, sb, sb2, dscale, nt);
+}
+
+// FP32 on pairs of column tiles (mfma_autodiff_rr2_dev; qp * 4 a multiple of
+// 64); the same arguments as k_mfma_autodiff_rr.  Three waves per SIMD: the
+// register budget that keeps them (168) also keeps the G accumulators and the
+// group's operands out of the AGPRs.
+template <int DM, int GR>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_mfma_autodiff_rr2(
+    const int* __restrict__ rp, const int* __restrict__ ci, const float* __restrict__ hv,
+    const float* __restrict__ am, const float* __restrict__ am1, const float* __restrict__ bm1,
+    const float* __restrict__ bm2, float* __restrict__ a_out, float* __restrict__ b_out,
+    double* __restrict__ dacc, int nb, int qp, int chunk, double c4, double c2, double w,
+    double sb, double sb2, double dscale, int nt) {
+  (void)nt;
+  mfma_autodiff_rr2_dev<DM, GR>(wg_self(), rp, ci, hv, am, am1, bm1, bm2, a_out, b_out, dacc, nb,
+                                qp, chunk, c4, c2, w, sb, sb2, dscale);
 }
 
 }  // namespace fkd

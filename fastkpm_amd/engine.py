@@ -338,10 +338,13 @@ class HipEngine:
         of what is free)."""
         call("fkpm_set_option", self._h, _capi.OPT_ALPHA_TABLE_MB, int(mib))
 
-    def set_mfma(self, on: bool) -> None:
+    def set_mfma(self, on: bool | int) -> None:
         """Matrix-core kernels where the block shape has them (complex 8x8
-        blocks; on by default), or the vector-unit kernels only."""
-        call("fkpm_set_option", self._h, _capi.OPT_MFMA, 1 if on else 0)
+        blocks; on by default), or the vector-unit kernels only (False); 2:
+        matrix cores with the row-resident kernels as the untuned choice of
+        every step that has one (FKPM_OPT_MFMA)."""
+        mode = on if isinstance(on, int) and not isinstance(on, bool) else (1 if on else 0)
+        call("fkpm_set_option", self._h, _capi.OPT_MFMA, int(mode))
 
     def kernel_config(self, kind: int) -> dict:
         v = [C.c_int() for _ in range(4)]

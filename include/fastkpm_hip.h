@@ -180,7 +180,10 @@ int fkpm_moments_noalloc(int device, int prec, int nb, int b, int nnzb, const in
                                    (default 0; the environment variable FKPM_AUTOTUNE=1 at
                                    fkpm_dev_open sets 1) */
 #define FKPM_OPT_MFMA 3         /* value 1 (default): matrix-core (MFMA) kernels where the block
-                                   shape has them (complex 8x8 blocks); 0: vector-unit kernels */
+                                   shape has them (complex 8x8 blocks); 2: the same, with the
+                                   row-resident variants (rows of at most 8 blocks) as the
+                                   untuned choice of every step that has one; 0: vector-unit
+                                   kernels */
 #define FKPM_OPT_STORE_ALPHA 5  /* value 2: fkpm_moments keeps every alpha_k when the table
                                    fits half the free device memory (or the byte count in the
                                    environment variable FKPM_ALPHA_TABLE_BYTES), and

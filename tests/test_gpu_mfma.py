@@ -170,6 +170,30 @@ def test_mfma_fp32_mixed(oracle):
     assert rel_err(D.val, orc.stoch_matrix(c, D)) < 1e-4
 
 
+@pytest.mark.parametrize("S_cols,slots", [(8, 7), (24, 7), (24, 8), (40, 8)])
+def test_mfma_fp32_row_resident_autodiff(oracle, S_cols, slots):
+    """The FP32 row-resident autodiff step (FKPM_OPT_MFMA 2) on single
+    16-column tiles (S = 8: a padded width of 8 packets) and on pairs of tiles
+    (S = 24, 40: 16 and 32 packets), with 7 and 8 block slots per row,
+    against the FP64 oracle."""
+    from fastkpm_amd import lattice
+    from helpers import with_partner_bonds
+    shape = (4, 4, 4)
+    H = lattice.lattice(shape, b=8, orbital_blocks=True, seed=5)
+    if slots == 8:
+        H = with_partner_bonds(H, shape, 6)
+    assert int(np.max(np.diff(H.row_ptr))) == slots
+    n = H.n_rows * H.b_len
+    R = oracle.set_R_uncorrelated(n, S_cols, oracle.MT19937(3 + S_cols))
+    eng, orc, lo, hi = _setup(oracle, H, R, prec="fp32", mfma=2)
+    eng.moments(2)
+    assert eng.kernel_config(K_AD)["mfma_variant"] == 2
+    packets = eng.config()["s_padded"] // 2  # two complex columns per 16-byte packet
+    assert (packets % 16 == 0) == (S_cols > 16)
+    _check_all(oracle, eng, orc, H, lo, hi, 40, 2e-5, 1e-4)
+    eng.close()
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_mfma_autotuned(oracle, prec):
     H = _lattice_b8(shape=(4, 4, 4), seed=5)

@@ -289,7 +289,7 @@ def test_stoch_matrix_from_table(oracle, prec, name, shape, b, J, W, ob, periods
 def test_option_refusals():
     from fastkpm_amd._capi import FkpmError, OPT_SWEEP, OPT_MFMA, call
     eng = _engine()
-    for opt, val in ((OPT_SWEEP, 65), (OPT_SWEEP, -2), (OPT_MFMA, 2), (99, 0)):
+    for opt, val in ((OPT_SWEEP, 65), (OPT_SWEEP, -2), (OPT_MFMA, 3), (99, 0)):
         with pytest.raises(FkpmError):
             call("fkpm_set_option", eng.handle, opt, val)
 
