@@ -535,7 +535,9 @@ const int* rp_for(const fkpm_handle* h, const KCfg& c) {
 
 // Kernel entry of a kind under its configuration.
 template <class S>
-auto mom_fn(const fkd::KSet<S>& ks, const KCfg& c) {
+auto mom_fn(const fkpm_handle* h, const fkd::KSet<S>& ks, const KCfg& c) {
+  if (c.mfma == 2 && ks.mfma_mom_rr2 != nullptr && h->qp % 16 == 0)
+    return c.dm == 7 ? ks.mfma_mom_rr2_7 : ks.mfma_mom_rr2;
   return c.mfma == 2 ? (c.dm == 7 ? ks.mfma_mom_rr7 : ks.mfma_mom_rr) : c.mfma ? ks.mfma_mom : ks.mom;
 }
 // The row-resident autodiff step in FP32 takes column tiles in pairs when the
@@ -808,7 +810,7 @@ int tune_t(fkpm_handle* h) {
     const auto& ks = h->ks<S>(K_MOM);
     S* outs[] = {A0, A1, A2, B0, B1, B2};
     for (S* o : outs)
-      CHK(launch(mom_fn(ks, c), dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R, (const S*)nullptr,
+      CHK(launch(mom_fn(h, ks, c), dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R, (const S*)nullptr,
                  o, h->nb, h->qp, c.chunk, 1.0, 0.0, (double*)nullptr, 0, no_rsource<S>()));
     CHK(zero_doubles(h, h->tune_dacc.p, (size_t)h->nnzb * h->b * h->b * HS));
   }
@@ -842,7 +844,7 @@ int tune_t(fkpm_handle* h) {
           const KCfg& c = h->kc[K_MOM];
           const auto& ks = h->ks<S>(K_MOM);
           for (int k = 0; k < ns; k++)  // every slot = Hs R
-            CHK(launch(mom_fn(ks, c), dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R,
+            CHK(launch(mom_fn(h, ks, c), dim3(c.nwg), dim3(fkd::WG), h->stream, rp, ci, hv, R,
                        (const S*)nullptr, slot(k), h->nb, h->qp, c.chunk, 1.0, 0.0,
                        (double*)nullptr, 0, no_rsource<S>()));
         }
@@ -904,7 +906,7 @@ int tune_t(fkpm_handle* h) {
           auto run = [&]() -> int {
             if (kind == K_MOMT) {  // alpha_{t+2} = 2 Hs alpha_{t+1} - alpha_t into a fresh slot
               const int t = tstep++;
-              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
+              return launch(mom_fn(h, ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
                             (const S*)slot((t + 1) % ns), (const S*)slot(t % ns),
                             slot((t + 2) % ns), h->nb, h->qp, k.chunk, 2.0, -1.0,
                             h->partial.as<double>(), k.nt, no_rsource<S>());
@@ -928,7 +930,7 @@ int tune_t(fkpm_handle* h) {
             // launch here is one, so both forms weigh in the choice
             if (kind == K_MOM) {
               const bool cl = t & 1;
-              return launch(mom_fn(ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
+              return launch(mom_fn(h, ks, k), dim3(k.nwg), dim3(fkd::WG), h->stream, rp_for(h, k), ci, hv,
                             (const S*)x, (const S*)z, y, h->nb, h->qp, k.chunk, 2.0, -1.0,
                             cl ? (double*)nullptr : h->partial.as<double>(), k.nt,
                             cl ? rsource<S>(h, 0.25) : no_rsource<S>());
@@ -1037,7 +1039,7 @@ int moments_t(fkpm_handle* h, int M, double* mu) {
       const S* x = (const S*)h->vec_a(m);
       const S* z = m == 0 ? nullptr : (const S*)h->vec_a(m - 1);
       S* y = (S*)h->vec_a(m + 1);
-      CHK(launch(mom_fn(ks, kc), grid, block, h->stream, rp_for(h, kc),
+      CHK(launch(mom_fn(h, ks, kc), grid, block, h->stream, rp_for(h, kc),
                  h->ci.as<const int>(), h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk,
                  m == 0 ? 1.0 : 2.0, m == 0 ? 0.0 : -1.0,
                  h->partial.as<double>() + (size_t)m * kc.nwg * 2, kc.nt, no_rsource<S>()));
@@ -1114,7 +1116,7 @@ int stoch_clenshaw(fkpm_handle* h, int M, const double* c, const S** xi_out) {
   Vb b1{R, c[M - 1]}, b2{nullptr, 0.0};
   int slot = 0, launches = 0;
   auto step = [&](double xfac, double cr, S* y) -> int {
-    return launch(mom_fn(ks, kc), grid, block, h->stream, rp_for(h, kc), h->ci.as<const int>(),
+    return launch(mom_fn(h, ks, kc), grid, block, h->stream, rp_for(h, kc), h->ci.as<const int>(),
                   h->hv.as<const S>(), b1.p, b2.p, y, h->nb, h->qp, kc.chunk, xfac * b1.s,
                   b2.p ? -b2.s : 0.0, (double*)nullptr, kc.nt, rsource<S>(h, cr));
   };
@@ -1179,7 +1181,7 @@ int stoch_table(fkpm_handle* h, int M, const double* c, const S** xi_out) {
     const bool last = (k == K - 1);
     const double cx = (k == 0 ? 1.0 : 2.0) * (last ? 2.0 : 1.0);
     const double cz = k == 0 ? 0.0 : (last ? -2.0 : -1.0);
-    CHK(launch(mom_fn(ks, kc), grid, block, h->stream, rp_for(h, kc),
+    CHK(launch(mom_fn(h, ks, kc), grid, block, h->stream, rp_for(h, kc),
                h->ci.as<const int>(), h->hv.as<const S>(), x, z, y, h->nb, h->qp, kc.chunk, cx,
                cz, (double*)nullptr, kc.nt, last ? addv : no_rsource<S>()));
     z = x;

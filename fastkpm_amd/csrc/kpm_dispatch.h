@@ -61,8 +61,12 @@ struct KSet {
   void (*mfma_ad_rr7)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
                       S*, S*, double*, int, int, int, double, double, double, double, double,
                       double, int) = nullptr;
-  // FP32 only: the row-resident autodiff step on pairs of column tiles (the
-  // padded width a multiple of 16 packets), 8 and 7 block slots
+  // FP32 only: the row-resident forward and autodiff steps on pairs of column
+  // tiles (the padded width a multiple of 16 packets), 8 and 7 block slots
+  void (*mfma_mom_rr2)(const int*, const int*, const S*, const S*, const S*, S*, int, int, int,
+                       double, double, double*, int, RSource<S>) = nullptr;
+  void (*mfma_mom_rr2_7)(const int*, const int*, const S*, const S*, const S*, S*, int, int, int,
+                         double, double, double*, int, RSource<S>) = nullptr;
   void (*mfma_ad_rr2)(const int*, const int*, const S*, const S*, const S*, const S*, const S*,
                       S*, S*, double*, int, int, int, double, double, double, double, double,
                       double, int) = nullptr;

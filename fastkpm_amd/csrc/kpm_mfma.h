@@ -427,6 +427,115 @@ __device__ __forceinline__ void mfma_step_rr_dev(
   if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
 }
 
+// FP32 row-resident forward / Clenshaw step on pairs of column tiles (same
+// contract as mfma_step_rr_dev<float>; qp * 4 a multiple of 64).  Lane (q, c)
+// takes columns 2c and 2c + 1 of a pair of 16-column tiles with one 16-byte
+// access per orbital row (q and q + 4) and feeds them to two accumulators, so
+// each group of neighbour loads serves 32 columns.
+template <int DM, int G>
+__device__ __forceinline__ void mfma_step_rr2_dev(
+    WgId wid, const int* __restrict__ rp, const int* __restrict__ ci,
+    const float* __restrict__ hv, const float* __restrict__ x, const float* __restrict__ z,
+    float* __restrict__ y, int nb, int qp, int chunk, double cx, double cz,
+    double* __restrict__ partial, RSource<float> rsrc) {
+  using S = float;
+  using C2 = Mfma<float>::c2;
+  using V4 = Mfma<float>::v4;
+  constexpr int NS = Vec16<S>::N;
+  constexpr uint32_t ES = sizeof(S);
+  const MfmaLane m = mfma_lane<S>();
+  const int g = threadIdx.x / WAVE;
+  const RowRange rr = wg_rows(nb, chunk, WG / WAVE, wid);
+  const uint32_t rs = (uint32_t)qp * NS;
+  const int npair = (int)(rs / 64);
+  const uint32_t vbytes = 8u * (uint32_t)nb * rs * ES;
+  const __amdgpu_buffer_rsrc_t bx = vec_rsrc(x, vbytes), by = vec_rsrc(y, vbytes);
+  const __amdgpu_buffer_rsrc_t bz = vec_rsrc(z != nullptr ? z : x, z != nullptr ? vbytes : 0u);
+  const S acx = (S)cx, acz = (S)cz;
+  const uint32_t half = 4u * rs * ES;
+  const uint32_t vlo = (m.q * rs + 4u * m.c) * ES;  // columns 2c, 2c + 1 of orbital q
+  double dmm = 0.0, dm1 = 0.0;
+
+  for (int base = rr.r0; base < rr.r1; base += rr.stride) {
+    const int i = __builtin_amdgcn_readfirstlane(base + g);
+    if (i >= rr.r1) continue;
+    const int p0 = rp[i], deg = rp[i + 1] - p0;
+    S a[DM][4];
+    uint32_t sj[DM];
+    if (deg > 0) {
+      int jj[DM];
+      HRaw<S> hr[DM];
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        const int pu = p0 + min(u, deg - 1);
+        jj[u] = ci[pu];
+        hr[u] = mfma_hraw<S>(hv + (size_t)pu * 128, m);
+      }
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+        mfma_a_of<S>(hr[u], m, u < deg ? S(1) : S(0), a[u]);
+        sj[u] = (uint32_t)__builtin_amdgcn_readfirstlane(8 * jj[u]) * rs * ES;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < DM; u++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[u][k] = S(0);
+        sj[u] = 0u;
+      }
+    }
+    const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 4u * m.c) * ES;
+    for (int t = 0; t < npair; t++) {
+      const uint32_t so = (uint32_t)t * 64u * ES;
+      V4 acc[2] = {V4{0.f, 0.f, 0.f, 0.f}, V4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int u0 = 0; u0 < DM; u0 += G) {
+        bf32x4 x0[G], x1[G];
+#pragma unroll
+        for (int v = 0; v < G; v++) {
+          if (u0 + v >= DM) continue;  // DM % G != 0: the last group is short
+          x0[v] = fkd_raw_ld_f32x4(bx, (int)vlo, (int)(sj[u0 + v] + so), 0);
+          x1[v] = fkd_raw_ld_f32x4(bx, (int)vlo, (int)(sj[u0 + v] + so + half), 0);
+        }
+#pragma unroll
+        for (int v = 0; v < G; v++) {
+          if (u0 + v >= DM) continue;
+          acc[0] = mfma_block<S>(a[u0 + v], C2{x0[v].x, x0[v].y}, C2{x1[v].x, x1[v].y}, acc[0]);
+          acc[1] = mfma_block<S>(a[u0 + v], C2{x0[v].z, x0[v].w}, C2{x1[v].z, x1[v].w}, acc[1]);
+        }
+      }
+      const bf32x4 xo0 = fkd_raw_ld_f32x4(bx, (int)f0, (int)so, 0);
+      const bf32x4 xo1 = fkd_raw_ld_f32x4(bx, (int)(f0 + half), (int)so, 0);
+      const bf32x4 z0 = fkd_raw_ld_f32x4(bz, (int)f0, (int)so, 0);  // 0 without z
+      const bf32x4 z1 = fkd_raw_ld_f32x4(bz, (int)(f0 + half), (int)so, 0);
+      C2 y0[2], y1[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        y0[h].x = fma(acx, acc[h].x, acz * z0[2 * h]);
+        y0[h].y = fma(acx, acc[h].y, acz * z0[2 * h + 1]);
+        y1[h].x = fma(acx, acc[h].z, acz * z1[2 * h]);
+        y1[h].y = fma(acx, acc[h].w, acz * z1[2 * h + 1]);
+        if (rsrc.cr != 0.0) {
+          const int col = t * 32 + 2 * m.c + h;
+          mfma_rsource<S>(rsrc, (size_t)(8 * i + m.q), col, (f0 + so) / ES + 2 * h, y0[h]);
+          mfma_rsource<S>(rsrc, (size_t)(8 * i + 4 + m.q), col, (f0 + half + so) / ES + 2 * h,
+                          y1[h]);
+        }
+      }
+      fkd_raw_st_f32x4(bf32x4{y0[0].x, y0[0].y, y0[1].x, y0[1].y}, by, (int)f0, (int)so, 0);
+      fkd_raw_st_f32x4(bf32x4{y1[0].x, y1[0].y, y1[1].x, y1[1].y}, by, (int)(f0 + half), (int)so, 0);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const double a0 = xo0[2 * h], a1 = xo0[2 * h + 1], a2 = xo1[2 * h], a3 = xo1[2 * h + 1];
+        dmm = fma(a0, a0, fma(a1, a1, fma(a2, a2, fma(a3, a3, dmm))));
+        dm1 = fma((double)y0[h].x, a0,
+                  fma((double)y0[h].y, a1, fma((double)y1[h].x, a2, fma((double)y1[h].y, a3, dm1))));
+      }
+    }
+  }
+  if (partial != nullptr) wg_sum2(dmm, dm1, partial + 2 * (size_t)wid.k);
+}
+
 // D(a, b) += fD * (G[a][b] + G[8+a][8+b], G[8+a][b] - G[a][8+b]) of block p from
 // the accumulator G in the MFMA output layout of S (see the file comment).
 template <typename S>
@@ -1053,6 +1162,17 @@ __global__ __launch_bounds__(WG) void k_mfma_autodiff_rr(
   mfma_autodiff_rr_dev<S, true, DM, mf_rr_group<S, DM>()>(wg_self(), rp, ci, hv, am, am1, bm1, bm2,
                                                          a_out, b_out, dacc, nb, qp, chunk, c4, c2,
                                                          w, sb, sb2, dscale, nt);
+}
+
+// FP32 forward / Clenshaw step on pairs of column tiles (mfma_step_rr2_dev);
+// the same arguments as k_mfma_step_rr.
+template <int DM>
+__global__ __launch_bounds__(WG) void k_mfma_step_rr2(
+    const int* __restrict__ rp, const int* __restrict__ ci, const float* __restrict__ hv,
+    const float* __restrict__ x, const float* __restrict__ z, float* __restrict__ y, int nb, int qp,
+    int chunk, double cx, double cz, double* __restrict__ partial, int nt, RSource<float> rsrc) {
+  (void)nt;
+  mfma_step_rr2_dev<DM, 4>(wg_self(), rp, ci, hv, x, z, y, nb, qp, chunk, cx, cz, partial, rsrc);
 }
 
 // FP32 on pairs of column tiles (mfma_autodiff_rr2_dev; qp * 4 a multiple of
