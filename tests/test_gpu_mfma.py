@@ -194,6 +194,40 @@ def test_mfma_fp32_row_resident_autodiff(oracle, S_cols, slots):
     eng.close()
 
 
+@pytest.mark.parametrize("shape", [(64, 64), (16, 16, 16)])
+def test_mfma_fp32_pairs_of_tiles_at_4096_rows(shape):
+    """FP32 row-resident forward and autodiff steps on pairs of column tiles
+    (S = 32 and 64 probing colours) on 4096 block rows of a 2-D (5 blocks a
+    row: two padding slots) and a 3-D lattice, against the FP64 engine:
+    moments within 2e-5, energy within 1e-6, dE/dH within 1e-4."""
+    from fastkpm_amd import HipEngine, host, lattice
+    from fastkpm_amd.serving import gershgorin
+    H = lattice.lattice(shape, b=8, orbital_blocks=True, seed=12)
+    assert H.n_rows == 4096
+    groups = lattice.colouring(shape, (2,) * len(shape), 8)
+    R = host.set_R_correlated("cx_double", groups, host.RNG(12))
+    es = gershgorin(H)
+    M = 64
+    c = host.expansion_coefficients(M, 2 * M, host.FERMI_ENERGY, 0.05, 0.0, es.lo, es.hi)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        eng = HipEngine("cx_double", prec)
+        eng.set_R(R)
+        eng.set_H(H, es)
+        mu = eng.moments(M)
+        assert eng.kernel_config(K_MOM)["mfma_variant"] == 2
+        assert eng.kernel_config(K_AD)["mfma_variant"] == 2
+        D = H.like()
+        eng.autodiff_matrix(c, D)
+        out[prec] = (mu, float(np.dot(c, mu)), D.val.copy())
+        if prec == "fp32":  # two complex columns per packet, 16 packets per pair of tiles
+            assert eng.config()["s_padded"] // 2 % 16 == 0
+        eng.close()
+    assert rel_err(out["fp32"][0], out["fp64"][0]) < 2e-5
+    assert abs(out["fp32"][1] - out["fp64"][1]) < 1e-6 * abs(out["fp64"][1])
+    assert rel_err(out["fp32"][2], out["fp64"][2]) < 1e-4
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_mfma_autotuned(oracle, prec):
     H = _lattice_b8(shape=(4, 4, 4), seed=5)
