@@ -194,6 +194,23 @@ def test_mfma_fp32_row_resident_autodiff(oracle, S_cols, slots):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_mfma_forced_row_resident_falls_back_on_long_rows(oracle, prec):
+    """FKPM_OPT_MFMA 2 asks for the row-resident kernels; rows of more than 8
+    blocks have none, so every kind keeps the general matrix-core kernels and
+    the results stay those of the oracle."""
+    H = _dense_rows_b8(nb=16, extra=5, seed=4)
+    assert int(np.max(np.diff(H.row_ptr))) > 8
+    R = _probing_R(oracle, H, seed=9)
+    eng, orc, lo, hi = _setup(oracle, H, R, prec=prec, mfma=2)
+    eng.moments(2)
+    assert eng.kernel_config(K_MOM)["mfma_variant"] == 1
+    assert eng.kernel_config(K_AD)["mfma_variant"] == 1
+    tol = (FP64_TOL, FP64_TOL) if prec == "fp64" else (2e-5, 1e-4)
+    _check_all(oracle, eng, orc, H, lo, hi, 32, *tol)
+    eng.close()
+
+
 @pytest.mark.parametrize("shape", [(64, 64), (16, 16, 16)])
 def test_mfma_fp32_pairs_of_tiles_at_4096_rows(shape):
     """FP32 row-resident forward and autodiff steps on pairs of column tiles
