@@ -485,6 +485,21 @@ __device__ __forceinline__ void mfma_step_rr2_dev(
       }
     }
     const uint32_t f0 = ((uint32_t)(8 * i + m.q) * rs + 4u * m.c) * ES;
+    // compact R (one nonzero per orbital row): the lane's two rows' column and
+    // value, loaded once per block row rather than per tile
+    const bool rcomp = rsrc.cr != 0.0 && rsrc.rd == nullptr;
+    int rc[2] = {-1, -1};
+    S rv[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+    if (rcomp) {
+#pragma unroll
+      for (int o = 0; o < 2; o++) {
+        const size_t row = (size_t)(8 * i + 4 * o + m.q);
+        rc[o] = rsrc.rcol[row];
+        rv[o][0] = rsrc.rval[2 * row];
+        rv[o][1] = rsrc.rval[2 * row + 1];
+      }
+    }
+    const S acr = (S)rsrc.cr;
     for (int t = 0; t < npair; t++) {
       const uint32_t so = (uint32_t)t * 64u * ES;
       V4 acc[2] = {V4{0.f, 0.f, 0.f, 0.f}, V4{0.f, 0.f, 0.f, 0.f}};
@@ -515,8 +530,17 @@ __device__ __forceinline__ void mfma_step_rr2_dev(
         y0[h].y = fma(acx, acc[h].y, acz * z0[2 * h + 1]);
         y1[h].x = fma(acx, acc[h].z, acz * z1[2 * h]);
         y1[h].y = fma(acx, acc[h].w, acz * z1[2 * h + 1]);
-        if (rsrc.cr != 0.0) {
-          const int col = t * 32 + 2 * m.c + h;
+        const int col = t * 32 + 2 * m.c + h;
+        if (rcomp) {
+          if (rc[0] == col) {
+            y0[h].x = fma(acr, rv[0][0], y0[h].x);
+            y0[h].y = fma(acr, rv[0][1], y0[h].y);
+          }
+          if (rc[1] == col) {
+            y1[h].x = fma(acr, rv[1][0], y1[h].x);
+            y1[h].y = fma(acr, rv[1][1], y1[h].y);
+          }
+        } else if (rsrc.cr != 0.0) {  // dense R
           mfma_rsource<S>(rsrc, (size_t)(8 * i + m.q), col, (f0 + so) / ES + 2 * h, y0[h]);
           mfma_rsource<S>(rsrc, (size_t)(8 * i + 4 + m.q), col, (f0 + half + so) / ES + 2 * h,
                           y1[h]);
